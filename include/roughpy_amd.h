@@ -1,0 +1,271 @@
+/*
+ * roughpy_amd.h -- C-ABI of the MI355X (gfx950) dense free-tensor library.
+ *
+ * Every entry point takes plain pointers and sizes.  Tensor/Lie data
+ * pointers are DEVICE pointers (HBM) unless the comment says "host".  The
+ * caller owns every buffer; the library allocates only its internal
+ * workspace (a grow-only per-device pool, freed by rpy_release_workspace).
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Return value: RPY_OK (0) or an rpy_status error code; rpy_last_error()
+ * returns a thread-local message for the last failure on this thread.
+ *
+ * Dense layout (roughpy_compute/README.md:110-134): level k of a tensor
+ * occupies [db[k], db[k+1]) with db[0]=0, db[k+1]=1+width*db[k]; the word
+ * (i_1..i_k) sits at db[k] + sum_j i_j width^(k-j).  A batch is a leading
+ * dimension: element e of operand X starts at X + e*X_stride (in elements).
+ */
+#ifndef ROUGHPY_AMD_H
+#define ROUGHPY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rpy_status {
+    RPY_OK = 0,
+    RPY_ERR_INVALID_ARGUMENT = 1, /* ValueError in the Python layer */
+    RPY_ERR_SHAPE = 2,            /* ValueError */
+    RPY_ERR_DTYPE = 3,            /* TypeError */
+    RPY_ERR_BASIS = 4,            /* ValueError */
+    RPY_ERR_HIP = 5,              /* RuntimeError */
+    RPY_ERR_OUT_OF_MEMORY = 6,    /* MemoryError */
+    RPY_ERR_UNSUPPORTED = 7       /* NotImplementedError */
+} rpy_status;
+
+/* Arithmetic type of a call.  RPY_F32_COMPENSATED reads and writes float
+ * data but carries each signature coefficient as a compensated pair. */
+typedef enum rpy_dtype {
+    RPY_F32 = 0,
+    RPY_F64 = 1,
+    RPY_F32_COMPENSATED = 2
+} rpy_dtype;
+
+/* Mirror of rpy::compute::BasisBase (roughpy_compute/common/basis.hpp:8-18)
+ * for a tensor basis.  degree_begin is a HOST array of depth+2 entries; it
+ * may be NULL, in which case the standard one is used (and checked when
+ * given). */
+typedef struct rpy_tensor_basis {
+    int32_t width;
+    int32_t depth;
+    const int64_t *degree_begin;
+} rpy_tensor_basis;
+
+/* Mirror of the per-operand DegreeBounds of the reference bindings
+ * (roughpy/compute/_src/call_config.hpp:11-15): max_degree -1 = basis depth. */
+typedef struct rpy_degree_bounds {
+    int32_t min_degree;
+    int32_t max_degree;
+} rpy_degree_bounds;
+
+/* Mirror of rpy::compute::LieBasis (basis.hpp:64-81): degree_begin (depth+2)
+ * and data ((size+1) x 2 parent pairs, row 0 = (0,0), letters (0,l)); both
+ * HOST arrays. */
+typedef struct rpy_lie_basis {
+    int32_t width;
+    int32_t depth;
+    const int64_t *degree_begin;
+    const int64_t *data;
+} rpy_lie_basis;
+
+/* Compressed sparse matrix, mirror of CompressedMatrixData
+ * (roughpy/compute/_src/dense_basic.cpp:554-562).  All arrays are DEVICE
+ * arrays when passed to a compute entry point.  compressed_rows != 0: CSR
+ * (n_offsets = rows), otherwise CSC (n_offsets = cols).  data has the call's
+ * dtype. */
+typedef struct rpy_sparse_matrix {
+    int64_t rows;
+    int64_t cols;
+    int64_t nnz;
+    const int64_t *offsets; /* n_offsets + 1 */
+    const int64_t *indices; /* nnz */
+    const void *data;       /* nnz */
+    int32_t compressed_rows;
+} rpy_sparse_matrix;
+
+/* ---- library ---------------------------------------------------------- */
+
+const char *rpy_last_error(void);
+int32_t rpy_version(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int32_t rpy_device_count(void);
+/* Frees the internal workspace pool of the current device. */
+int rpy_release_workspace(void);
+
+/* ---- bases (host) ------------------------------------------------------- */
+
+/* Size of a truncated tensor algebra; -1 on overflow. */
+int64_t rpy_tensor_size(int32_t width, int32_t depth);
+
+/* Lie basis size (sum of Witt dimensions), mirror of compute_lie_dim
+ * (roughpy/compute/_src/lie_basis.c:1804-1813). */
+int64_t rpy_lie_basis_size(int32_t width, int32_t depth);
+
+/* Builds the greedy Hall set of LieBasis(width, depth)
+ * (roughpy/compute/_src/lie_basis.c:201-286).  degree_begin: host, depth+2;
+ * data: host, (size+1)*2.  */
+int rpy_lie_basis_build(int32_t width, int32_t depth, int64_t *degree_begin,
+                        int64_t *data);
+
+/* Validity check of user Hall-set data, Bourbaki major, mirror of
+ * check_lie_basis_data (lie_basis.c:1333-1419, 1651-1682).  Returns RPY_OK
+ * when valid, RPY_ERR_BASIS with the reason in rpy_last_error otherwise. */
+int rpy_check_lie_basis_data(const rpy_lie_basis *basis);
+
+/* l2t map as CSR (tensor_dim rows x lie_dim cols), float64 values, host
+ * arrays; mirror of LieBasis.get_l2t_matrix (lie_basis.c:902-994).  Call
+ * with NULL arrays to query nnz (written to *nnz). */
+int rpy_l2t_matrix(const rpy_lie_basis *basis, int64_t *nnz, int64_t *offsets,
+                   int64_t *indices, double *data);
+
+/* t2l map as CSR (lie_dim rows x tensor_dim cols), float64, host arrays;
+ * mirror of LieBasis.get_t2l_matrix (lie_basis.c:1150-1278): Dynkin right
+ * bracketing expanded in the Hall basis and scaled by 1/degree. */
+int rpy_t2l_matrix(const rpy_lie_basis *basis, int64_t *nnz, int64_t *offsets,
+                   int64_t *indices, double *data);
+
+/* ---- dense free-tensor operations (device) ------------------------------ */
+
+/* out += lhs (x) rhs, truncated; replaces dense_ft_fma
+ * (roughpy/compute/_src/dense_basic.cpp:85-151, kernel
+ * roughpy_compute/dense/basic/free_tensor_fma.hpp:13-53). */
+int rpy_dense_ft_fma(void *out, const void *lhs, const void *rhs,
+                     rpy_dtype dtype, const rpy_tensor_basis *basis,
+                     rpy_degree_bounds out_bounds, rpy_degree_bounds lhs_bounds,
+                     rpy_degree_bounds rhs_bounds, int64_t batch,
+                     int64_t out_stride, int64_t lhs_stride,
+                     int64_t rhs_stride, void *stream);
+
+/* lhs <- lhs (x) rhs; replaces dense_ft_inplace_mul (dense_basic.cpp:
+ * 195-260, kernel free_tensor_inplace_mul.hpp:13-87). */
+int rpy_dense_ft_inplace_mul(void *lhs, const void *rhs, rpy_dtype dtype,
+                             const rpy_tensor_basis *basis,
+                             rpy_degree_bounds lhs_bounds,
+                             rpy_degree_bounds rhs_bounds, int64_t batch,
+                             int64_t lhs_stride, int64_t rhs_stride,
+                             void *stream);
+
+/* out = antipode(arg); replaces dense_ft_antipode (dense_basic.cpp:
+ * 295-360, kernel free_tensor_antipode.hpp:45-100). */
+int rpy_dense_ft_antipode(void *out, const void *arg, rpy_dtype dtype,
+                          const rpy_tensor_basis *basis, int64_t batch,
+                          int64_t out_stride, int64_t arg_stride,
+                          void *stream);
+
+/* out += L_op^*(arg) (adjoint of left multiplication); replaces
+ * dense_ft_adjoint_left_mul (roughpy/compute/_src/dense_basic.cpp:387-490,
+ * kernel free_tensor_adjoint_left_mul.hpp:10-77). */
+int rpy_dense_ft_adjoint_left_mul(void *out, const void *op, const void *arg,
+                                  rpy_dtype dtype,
+                                  const rpy_tensor_basis *basis,
+                                  int64_t batch, int64_t out_stride,
+                                  int64_t op_stride, int64_t arg_stride,
+                                  void *stream);
+
+/* out = exp(arg) (arg[0] ignored); replaces dense_ft_exp
+ * (roughpy/compute/_src/dense_intermediate.cpp:62-102, kernel
+ * free_tensor_exp.hpp:13-37). */
+int rpy_dense_ft_exp(void *out, const void *arg, rpy_dtype dtype,
+                     const rpy_tensor_basis *basis, int64_t batch,
+                     int64_t out_stride, int64_t arg_stride, void *stream);
+
+/* out = multiplier * exp(exponent) (exponent[0] ignored); replaces
+ * dense_ft_fmexp (dense_intermediate.cpp:153-208, kernel
+ * free_tensor_fmexp.hpp:15-40). */
+int rpy_dense_ft_fmexp(void *out, const void *multiplier, const void *exponent,
+                       rpy_dtype dtype, const rpy_tensor_basis *basis,
+                       int64_t batch, int64_t out_stride, int64_t mul_stride,
+                       int64_t exp_stride, void *stream);
+
+/* out = log(arg), assuming arg[0] == 1; replaces dense_ft_log
+ * (dense_intermediate.cpp:254-295, kernel free_tensor_log.hpp:12-40). */
+int rpy_dense_ft_log(void *out, const void *arg, rpy_dtype dtype,
+                     const rpy_tensor_basis *basis, int64_t batch,
+                     int64_t out_stride, int64_t arg_stride, void *stream);
+
+/* out[e] += scale * M arg[e] for a device CSR/CSC matrix; replaces
+ * dense_lie_to_tensor / dense_tensor_to_lie (dense_basic.cpp:862-1047,
+ * 1116-1298, kernel apply_sparse_linear_map.hpp:16-55).  out has
+ * matrix.rows entries per element, arg matrix.cols.  scale 1 = none. */
+int rpy_dense_sparse_map(void *out, const void *arg,
+                         const rpy_sparse_matrix *matrix, rpy_dtype dtype,
+                         double scale, int64_t batch, int64_t out_stride,
+                         int64_t arg_stride, void *stream);
+
+/* ---- streams (device) --------------------------------------------------- */
+
+/* Signatures of n_streams piecewise-linear streams of uneven length.
+ * increments: device, rows of `width` values, stream s owning rows
+ * [offsets[s], offsets[s+1]); offsets: device int64, n_streams+1.
+ * out: device, n_streams x tensor_size(width, depth).  Result of stream s is
+ * the Chen product of exp(row) over its rows, i.e. Context::compute_signature
+ * (algebra/src/lite_context.h:545-563) applied per stream, the value
+ * LieIncrementStream.signature() returns for the stream's full support
+ * (streams/src/stream.cpp:156-171).  dtype: RPY_F64 (double data),
+ * RPY_F32 or RPY_F32_COMPENSATED (float data).  Work is one persistent
+ * launch over a length-sorted queue. */
+int rpy_stream_signature_batch(void *out, const void *increments,
+                               const int64_t *offsets, int64_t n_streams,
+                               int32_t width, int32_t depth, rpy_dtype dtype,
+                               void *stream);
+
+/* Log-signatures in the Hall basis: t2l(log(signature)) per stream
+ * (lite_context.h:899-902; Stream.log_signature, stream.cpp:107-123).
+ * The log and the t2l projection run in float64 whatever `dtype` is; out is
+ * float64, n_streams x lie_size.  t2l: device CSR matrix from
+ * rpy_t2l_matrix (float64 values). */
+int rpy_stream_log_signature_batch(double *out, const void *increments,
+                                   const int64_t *offsets, int64_t n_streams,
+                                   int32_t width, int32_t depth,
+                                   rpy_dtype dtype,
+                                   const rpy_sparse_matrix *t2l,
+                                   void *stream);
+
+/* ---- stream queries (host) ---------------------------------------------- */
+
+/* -min(0, e-1) where arg = m 2^e, 0.5 <= |m| < 1; mirror of
+ * param_to_resolution (streams/include/roughpy/streams/stream_base.h:73-83). */
+int32_t rpy_param_to_resolution(double arg);
+
+/* Maximal aligned dyadic pieces k 2^-n of the clopen [inf, sup) at tolerance
+ * tol, left to right; mirror of to_dyadic_intervals
+ * (intervals/src/dyadic_interval.cpp:191-228).  *count receives the number of
+ * pieces; k_out/n_out (host, max_pieces entries) may be NULL to query it. */
+int rpy_to_dyadic_intervals(double inf, double sup, int32_t tol,
+                            int64_t *k_out, int32_t *n_out, int32_t max_pieces,
+                            int32_t *count);
+
+/* Rows of a LieIncrementStream that the query [inf, sup) at
+ * query_resolution covers, as [begin, end) ranges over the strictly
+ * increasing params (host).  Leaves at stream_resolution are whole;
+ * a dissection piece finer than the stream contributes its leaf only when it
+ * starts where the leaf starts and holds a row -- the semantics of
+ * DyadicCachingLayer::log_signature (streams/src/dyadic_caching_layer.cpp:
+ * 86-130) over LieIncrementStream leaves (streams/src/
+ * lie_increment_stream.cpp:174-199).  ranges: host, 2*max_ranges entries. */
+int rpy_stream_select_rows(const double *params, int64_t n_params,
+                           int32_t stream_resolution, double inf, double sup,
+                           int32_t query_resolution, int64_t *ranges,
+                           int64_t max_ranges, int64_t *n_ranges);
+
+/* ---- test hooks (host only; no counterpart in the reference) ----------- */
+
+/* Runs the lane decomposition of the specialised signature kernel on the
+ * host for one float64 stream of `length` increment rows (host arrays), with
+ * prefix length `prefix`.  Lets CPU-only tests check the kernel's index map. */
+int rpy_debug_signature_host_emulation(double *out, const double *increments,
+                                       int64_t length, int32_t width,
+                                       int32_t depth, int32_t prefix);
+
+/* Prefix length the signature kernel uses for (width, depth, dtype), or 0 when
+ * the shape runs on the generic kernel. */
+int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth,
+                                   int32_t is_f64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROUGHPY_AMD_H */

@@ -1,0 +1,611 @@
+"""CPU oracle for the dense free-tensor hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: tests/, the smoke
+check in __graft_entry__ and the cpu_baseline leg of bench.py import it.
+The product package (roughpy_amd) never imports it, and it never serves a
+product call.
+
+Contents, each restating the reference algorithm it cites:
+
+* ctypes bindings to liboracle.so (oracle.c / oracle_ops.h): the dense
+  free-tensor loops (ft_fma, ft_inplace_mul, ft_exp, ft_fmexp, ft_log,
+  antipode, adjoint of left multiplication, sparse map, Chen signature);
+* ctypes bindings to _ref/libref.so when it was built from the reference
+  headers (ref_harness.cpp) -- a second anchor;
+* numpy restatements of the Hall basis (greedy construction), the Witt
+  dimensions, lie_to_tensor (commutator expansion) and tensor_to_lie
+  (Dynkin right bracketing, solved independently by least squares);
+* the dyadic interval dissection and the stream query semantics of
+  LieIncrementStream / DyadicCachingLayer.
+
+Parity pins (what checks each function beyond retyping it) are listed in
+oracle/README.md and exercised by tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from functools import lru_cache
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_REF_PATH = os.path.join(_HERE, "_ref", "libref.so")
+
+_c_int = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_ptr = ctypes.c_void_p
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@lru_cache(maxsize=None)
+def lib() -> ctypes.CDLL:
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"oracle library missing at {_LIB_PATH}; run `make -C oracle`")
+    L = ctypes.CDLL(_LIB_PATH)
+    for sfx in ("f64", "f32"):
+        real = ctypes.c_double if sfx == "f64" else ctypes.c_float
+        f = getattr(L, f"orc_ft_fma_{sfx}")
+        f.argtypes = [_ptr, _ptr, _ptr, _c_int] + [_c_int] * 6 + [real]
+        f = getattr(L, f"orc_ft_inplace_mul_{sfx}")
+        f.argtypes = [_ptr, _ptr, _c_int] + [_c_int] * 4 + [real]
+        getattr(L, f"orc_ft_exp_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int]
+        getattr(L, f"orc_ft_fmexp_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int]
+        getattr(L, f"orc_ft_log_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int]
+        getattr(L, f"orc_antipode_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int]
+        getattr(L, f"orc_adj_lmul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int] + [_c_int] * 6
+        getattr(L, f"orc_signature_{sfx}").argtypes = [_ptr, _ptr, _c_i64, _c_int, _c_int, _ptr]
+        f = getattr(L, f"orc_signature_batch_{sfx}")
+        f.argtypes = [_ptr, _ptr, _ptr, _c_i64, _c_int, _c_int, _c_int]
+        f.restype = _c_int
+        f = getattr(L, f"orc_sparse_apply_{sfx}")
+        f.argtypes = [_ptr, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_int, real]
+    return L
+
+
+@lru_cache(maxsize=None)
+def ref_lib() -> Optional[ctypes.CDLL]:
+    """The reference harness (oracle/_ref/libref.so), or None if absent."""
+    if not os.path.exists(_REF_PATH):
+        return None
+    L = ctypes.CDLL(_REF_PATH)
+    for sfx in ("f64", "f32"):
+        getattr(L, f"ref_ft_fma_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_ft_inplace_mul_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_ft_exp_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_ft_fmexp_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_ft_log_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_antipode_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_adj_lmul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_signature_batch_{sfx}").argtypes = [
+            _ptr, _ptr, _ptr, _c_i64, _c_int, _c_int, _c_int]
+    return L
+
+
+def _sfx(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "f64"
+    if dt == np.float32:
+        return "f32"
+    raise TypeError(f"oracle supports float32/float64, not {dt}")
+
+
+# ---------------------------------------------------------------------------
+# Bases
+# ---------------------------------------------------------------------------
+
+def tensor_degree_begin(width: int, depth: int) -> np.ndarray:
+    """db[0]=0, db[k+1]=1+w*db[k] (roughpy/compute/_src/tensor_basis.c:42-58)."""
+    db = np.zeros(depth + 2, dtype=np.int64)
+    for k in range(depth + 1):
+        db[k + 1] = 1 + width * db[k]
+    return db
+
+
+def tensor_size(width: int, depth: int) -> int:
+    return int(tensor_degree_begin(width, depth)[-1])
+
+
+def _mobius(n: int) -> int:
+    result, p = 1, 2
+    while p * p <= n:
+        if n % p == 0:
+            n //= p
+            if n % p == 0:
+                return 0
+            result = -result
+        p += 1
+    return -result if n > 1 else result
+
+
+def witt_dim(width: int, degree: int) -> int:
+    """Number of Lie basis elements of one degree (lie_basis.c:1788-1802)."""
+    if degree <= 0:
+        return 0
+    if degree == 1:
+        return width
+    s = sum(_mobius(d) * width ** (degree // d)
+            for d in range(1, degree + 1) if degree % d == 0)
+    return s // degree
+
+
+def hall_basis(width: int, depth: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Greedy Hall set: (degree_begin[depth+2], data[size+1, 2]).
+
+    Restates roughpy/compute/_src/lie_basis.c:201-286 (same order as
+    vendored/libalgebra_lite/basis/hall_set.cpp:81-118): key 0 is (0, 0),
+    letters are (0, l), and for each degree, left degree ascending, a pair
+    (i, j) with i < j is kept iff the left parent of j is <= i.
+    """
+    data = [(0, 0)]
+    db = [0]
+    if depth > 0:
+        db.append(1)
+        for l in range(1, width + 1):
+            data.append((0, l))
+        db.append(len(data))
+    for degree in range(2, depth + 1):
+        for ld in range(1, degree // 2 + 1):
+            rd = degree - ld
+            for i in range(db[ld], db[ld + 1]):
+                for j in range(max(i + 1, db[rd]), db[rd + 1]):
+                    if data[j][0] <= i:
+                        data.append((i, j))
+        db.append(len(data))
+    return np.array(db, dtype=np.int64), np.array(data, dtype=np.int64).reshape(-1, 2)
+
+
+def hall_key_degree(db: np.ndarray, key: int) -> int:
+    for d in range(len(db) - 1):
+        if db[d] <= key < db[d + 1]:
+            return d
+    raise IndexError(key)
+
+
+def hall_key_str(data: np.ndarray, key: int) -> str:
+    a, b = int(data[key, 0]), int(data[key, 1])
+    if a == 0:
+        return str(b)
+    return f"[{hall_key_str(data, a)},{hall_key_str(data, b)}]"
+
+
+def _level_bracket(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """[x, y] = xy - yx for homogeneous word-vectors flattened by level."""
+    return np.kron(x, y) - np.kron(y, x)
+
+
+def lie_key_tensors(width: int, depth: int):
+    """Homogeneous tensor expansion of every Hall key (index 1..n)."""
+    db, data = hall_basis(width, depth)
+    out = [None]
+    for key in range(1, len(data)):
+        a, b = int(data[key, 0]), int(data[key, 1])
+        if a == 0:
+            v = np.zeros(width)
+            v[b - 1] = 1.0
+        else:
+            v = _level_bracket(out[a], out[b])
+        out.append(v)
+    return db, data, out
+
+
+def l2t_dense(width: int, depth: int) -> np.ndarray:
+    """Dense l2t matrix (tensor_dim x lie_dim): column key-1 is the
+    commutator expansion of the key (lie_basis.c:806-950)."""
+    db_t = tensor_degree_begin(width, depth)
+    db, data, vecs = lie_key_tensors(width, depth)
+    m = np.zeros((int(db_t[-1]), len(data) - 1))
+    for key in range(1, len(data)):
+        d = hall_key_degree(db, key)
+        m[db_t[d]:db_t[d + 1], key - 1] = vecs[key]
+    return m
+
+
+def dynkin_word(width: int, word: Sequence[int]) -> np.ndarray:
+    """r(a1..ak) = [a1, [a2, ... [a_{k-1}, a_k]]] as a homogeneous vector."""
+    def letter(a):
+        v = np.zeros(width)
+        v[a] = 1.0
+        return v
+    r = letter(word[-1])
+    for a in reversed(word[:-1]):
+        r = _level_bracket(letter(a), r)
+    return r
+
+
+def t2l_dense(width: int, depth: int) -> np.ndarray:
+    """Dense t2l matrix (lie_dim x tensor_dim).
+
+    Column of word u (degree k) is r(u)/k expanded in the Hall basis, the
+    definition used by lie_basis.c:1150-1250 (rbracketing then 1/deg,
+    :1066-1096) and libalgebra_lite maps.h:170-221.  The oracle finds the
+    expansion independently of the reference's bracket rewriting: it solves
+    l2t_k c = r(u) by least squares per degree and checks that the residual
+    vanishes and that k*c is integral.
+    """
+    db_t = tensor_degree_begin(width, depth)
+    db, data, vecs = lie_key_tensors(width, depth)
+    n_lie = len(data) - 1
+    out = np.zeros((n_lie, int(db_t[-1])))
+    for k in range(1, depth + 1):
+        keys = list(range(int(db[k]), int(db[k + 1])))
+        A = np.stack([vecs[key] for key in keys], axis=1)
+        nwords = width ** k
+        R = np.zeros((nwords, nwords))
+        for idx in range(nwords):
+            word, t = [], idx
+            for _ in range(k):
+                word.append(t % width)
+                t //= width
+            word.reverse()
+            R[:, idx] = dynkin_word(width, word)
+        C, *_ = np.linalg.lstsq(A, R, rcond=None)
+        resid = np.abs(A @ C - R).max() if R.size else 0.0
+        if resid > 1e-8:
+            raise AssertionError(f"Dynkin image not in Hall span at degree {k}")
+        Ci = np.rint(C)
+        if np.abs(C - Ci).max() > 1e-8:
+            raise AssertionError("non-integral Hall expansion")
+        out[np.array(keys) - 1, db_t[k]:db_t[k + 1]] = Ci / k
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Dense tensor operations (per element over leading batch dims)
+# ---------------------------------------------------------------------------
+
+def _batched(*arrays):
+    shape = arrays[0].shape[:-1]
+    n = int(np.prod(shape)) if shape else 1
+    return n, [a.reshape(n, a.shape[-1]) for a in arrays]
+
+
+def ft_fma(out, lhs, rhs, width, depth, out_bounds=(0, -1), lhs_bounds=(0, -1),
+           rhs_bounds=(0, -1), factor=1.0):
+    """out += lhs (x) rhs in place (free_tensor_fma.hpp:13-53)."""
+    L = lib()
+    f = getattr(L, f"orc_ft_fma_{_sfx(out.dtype)}")
+    ob = (out_bounds[0], depth if out_bounds[1] < 0 else out_bounds[1])
+    lb = (lhs_bounds[0], depth if lhs_bounds[1] < 0 else lhs_bounds[1])
+    rb = (rhs_bounds[0], depth if rhs_bounds[1] < 0 else rhs_bounds[1])
+    n, (o, l, r) = _batched(out, np.ascontiguousarray(lhs), np.ascontiguousarray(rhs))
+    assert o.base is out or o is out or np.shares_memory(o, out)
+    for e in range(n):
+        f(_p(o[e]), _p(l[e]), _p(r[e]), width, *ob, *lb, *rb, factor)
+    return out
+
+
+def ft_mul(lhs, rhs, width, depth):
+    out = np.zeros_like(lhs)
+    return ft_fma(out, lhs, rhs, width, depth)
+
+
+def ft_inplace_mul(lhs, rhs, width, depth, lhs_bounds=(0, -1), rhs_bounds=(0, -1),
+                   factor=1.0):
+    L = lib()
+    f = getattr(L, f"orc_ft_inplace_mul_{_sfx(lhs.dtype)}")
+    lb = (lhs_bounds[0], depth if lhs_bounds[1] < 0 else lhs_bounds[1])
+    rb = (rhs_bounds[0], depth if rhs_bounds[1] < 0 else rhs_bounds[1])
+    n, (l, r) = _batched(lhs, np.ascontiguousarray(rhs))
+    for e in range(n):
+        f(_p(l[e]), _p(r[e]), width, *lb, *rb, factor)
+    return lhs
+
+
+def _unary(name, arg, width, depth):
+    L = lib()
+    f = getattr(L, f"orc_{name}_{_sfx(arg.dtype)}")
+    arg = np.ascontiguousarray(arg)
+    out = np.zeros(arg.shape[:-1] + (tensor_size(width, depth),), dtype=arg.dtype)
+    n, (o, a) = _batched(out, arg)
+    for e in range(n):
+        f(_p(o[e]), _p(a[e]), width, depth)
+    return out
+
+
+def ft_exp(arg, width, depth):
+    return _unary("ft_exp", arg, width, depth)
+
+
+def ft_log(arg, width, depth):
+    return _unary("ft_log", arg, width, depth)
+
+
+def antipode(arg, width, depth):
+    return _unary("antipode", arg, width, depth)
+
+
+def ft_fmexp(mul, expo, width, depth):
+    L = lib()
+    f = getattr(L, f"orc_ft_fmexp_{_sfx(mul.dtype)}")
+    mul = np.ascontiguousarray(mul)
+    expo = np.ascontiguousarray(expo)
+    out = np.zeros_like(mul)
+    n, (o, m, x) = _batched(out, mul, expo)
+    for e in range(n):
+        f(_p(o[e]), _p(m[e]), _p(x[e]), width, depth)
+    return out
+
+
+def ft_adjoint_left_mul(op, arg, width, depth):
+    L = lib()
+    f = getattr(L, f"orc_adj_lmul_{_sfx(arg.dtype)}")
+    op = np.ascontiguousarray(op)
+    arg = np.ascontiguousarray(arg)
+    out = np.zeros_like(arg)
+    n, (o, p, a) = _batched(out, op, arg)
+    for e in range(n):
+        f(_p(o[e]), _p(p[e]), _p(a[e]), width, 0, depth, 0, depth, 0, depth)
+    return out
+
+
+def lie_to_tensor(arg, width, depth, scale=1.0):
+    return (np.asarray(arg, dtype=np.float64) @ l2t_dense(width, depth).T * scale).astype(arg.dtype)
+
+
+def tensor_to_lie(arg, width, depth, scale=1.0):
+    return (np.asarray(arg, dtype=np.float64) @ t2l_dense(width, depth).T * scale).astype(arg.dtype)
+
+
+# ---------------------------------------------------------------------------
+# Signatures
+# ---------------------------------------------------------------------------
+
+def signature(increments: np.ndarray, width: int, depth: int) -> np.ndarray:
+    """Chen signature of one stream (lite_context.h:545-563)."""
+    inc = np.ascontiguousarray(increments).reshape(-1, width)
+    L = lib()
+    sfx = _sfx(inc.dtype)
+    sz = tensor_size(width, depth)
+    out = np.zeros(sz, dtype=inc.dtype)
+    scratch = np.zeros(3 * sz, dtype=inc.dtype)
+    getattr(L, f"orc_signature_{sfx}")(_p(out), _p(inc), inc.shape[0], width, depth,
+                                       _p(scratch))
+    return out
+
+
+def signature_batch(increments: np.ndarray, offsets: np.ndarray, width: int,
+                    depth: int, nthreads: int = 1, use_ref: bool = False) -> np.ndarray:
+    """Signatures of uneven streams given CSR offsets over increment rows."""
+    inc = np.ascontiguousarray(increments).reshape(-1, width)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    ns = off.shape[0] - 1
+    sfx = _sfx(inc.dtype)
+    out = np.zeros((ns, tensor_size(width, depth)), dtype=inc.dtype)
+    if use_ref:
+        R = ref_lib()
+        if R is None:
+            raise RuntimeError("reference harness not built")
+        getattr(R, f"ref_signature_batch_{sfx}")(_p(out), _p(inc), _p(off), ns, width,
+                                                 depth, nthreads)
+    else:
+        rc = getattr(lib(), f"orc_signature_batch_{sfx}")(_p(out), _p(inc), _p(off), ns,
+                                                          width, depth, nthreads)
+        if rc != 0:
+            raise RuntimeError("oracle signature batch failed")
+    return out
+
+
+def log_signature(increments: np.ndarray, width: int, depth: int) -> np.ndarray:
+    """t2l(log(signature)) (lite_context.h:899-902)."""
+    s = signature(np.asarray(increments, dtype=np.float64), width, depth)
+    return tensor_to_lie(ft_log(s, width, depth), width, depth)
+
+
+def level_errors(x: np.ndarray, ref: np.ndarray, width: int, depth: int) -> np.ndarray:
+    """Normwise relative error per level: max|x_k - y_k| / max|y_k|."""
+    db = tensor_degree_begin(width, depth)
+    x = np.asarray(x, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    errs = []
+    for k in range(depth + 1):
+        a = x[..., db[k]:db[k + 1]]
+        b = ref[..., db[k]:db[k + 1]]
+        num = np.abs(a - b).max(axis=-1)
+        den = np.abs(b).max(axis=-1)
+        errs.append(np.where(den > 0, num / np.where(den > 0, den, 1), num))
+    return np.stack(errs, axis=-1)
+
+
+# ---------------------------------------------------------------------------
+# Dyadic intervals and stream query semantics
+# ---------------------------------------------------------------------------
+
+def param_to_resolution(arg: float) -> int:
+    """streams/include/roughpy/streams/stream_base.h:73-83."""
+    _, e = math.frexp(arg)
+    return -min(0, e - 1)
+
+
+class _DI:
+    """Clopen dyadic interval [k/2^n, (k+1)/2^n) (dyadic_interval.cpp:47-183)."""
+
+    __slots__ = ("k", "n")
+
+    def __init__(self, k: int, n: int):
+        self.k, self.n = k, n
+
+    @staticmethod
+    def from_dyadic(k: int, n: int, resolution: int) -> "_DI":
+        # rebase to `resolution`, rounding the multiplier down when coarser.
+        if resolution >= n:
+            return _DI(k << (resolution - n), resolution)
+        p = 1 << (n - resolution)
+        return _DI((k - (k % p)) // p, resolution)
+
+    @staticmethod
+    def from_param(val: float, resolution: int) -> "_DI":
+        return _DI(int(math.floor(math.ldexp(val, resolution))), resolution)
+
+    def expand(self, arg: int = 1) -> "_DI":
+        return _DI.from_dyadic(self.k, self.n, self.n - arg)
+
+    def aligned(self) -> bool:
+        parent = _DI.from_dyadic(self.k, self.n, self.n - 1)
+        back = _DI.from_dyadic(parent.k, parent.n, self.n)
+        return back.k == self.k and back.n == self.n
+
+    def contains(self, other: "_DI") -> bool:
+        if other.n >= self.n:
+            p = 1 << (other.n - self.n)
+            return (self.k << (other.n - self.n)) == other.k - (other.k % p)
+        return False
+
+    def shrink_to_contained_end(self, arg: int = 1) -> "_DI":
+        return _DI.from_dyadic(self.k, self.n, self.n + arg)
+
+    def flip(self) -> "_DI":
+        return _DI(self.k + 1 if self.k % 2 == 0 else self.k - 1, self.n)
+
+    def shrink_to_omitted_end(self) -> "_DI":
+        return self.shrink_to_contained_end().flip()
+
+    def inf(self) -> float:
+        return math.ldexp(self.k, -self.n)
+
+    def sup(self) -> float:
+        return math.ldexp(self.k + 1, -self.n)
+
+    def key(self):
+        return (self.k, self.n)
+
+
+def to_dyadic_intervals(inf: float, sup: float, tol: int) -> List[Tuple[int, int]]:
+    """Maximal aligned dyadic cover of clopen [inf, sup) at tolerance `tol`,
+    left to right, as (multiplier, power) pairs.
+    Restates intervals/src/dyadic_interval.cpp:191-228."""
+    begin = _DI.from_param(inf, tol)
+    end = _DI.from_param(sup, tol)
+    left: List[_DI] = []
+    right: List[_DI] = []
+    while not begin.contains(end):
+        nxt = begin.expand()
+        if not begin.aligned():
+            left.append(nxt.shrink_to_omitted_end())
+            nxt = _DI(nxt.k + 1, nxt.n)
+        begin = nxt
+    nxt = end
+    while True:
+        nxt = nxt.expand()
+        if not begin.contains(nxt):
+            break
+        if not end.aligned():
+            right.insert(0, nxt.shrink_to_contained_end())
+        end = nxt
+    return [d.key() for d in left + right]
+
+
+class IncrementStream:
+    """Oracle for LieIncrementStream queries.
+
+    Construction restates roughpy/src/streams/lie_increment_stream.cpp:
+    176-215 (default indices 0..N-1 with resolution 0; support
+    [min, max + 2^-res)) and streams/src/lie_increment_stream.cpp:116-159
+    (param -> increment map, first entry wins on duplicate params).  Queries
+    restate streams/src/stream.cpp:54-171 (trim, resolution refinement,
+    dyadic dissection) and streams/src/dyadic_caching_layer.cpp:86-130
+    (leaves at the stream resolution; finer pieces map to the leaf only when
+    they start where it starts).  Leaf and piece combination is the tensor
+    product of the pieces' signatures -- the reference combines them in log
+    space with CBH, which is the same element of the truncated group.
+    """
+
+    def __init__(self, data: np.ndarray, width: int, depth: int,
+                 indices: Optional[np.ndarray] = None,
+                 resolution: Optional[int] = None):
+        data = np.asarray(data, dtype=np.float64).reshape(-1, width)
+        self.width, self.depth = width, depth
+        n = data.shape[0]
+        if indices is None:
+            indices = np.arange(n, dtype=np.float64)
+            resolution = 0
+        indices = np.asarray(indices, dtype=np.float64)
+        if resolution is None:
+            # quirk kept: `previous` is never advanced, so this is min(idx).
+            min_diff = math.inf
+            for idx in indices:
+                if idx - 0.0 < min_diff:
+                    min_diff = idx - 0.0
+            resolution = param_to_resolution(min_diff) + 1
+        self.resolution = int(resolution)
+        self.params: List[float] = []
+        self.rows: List[np.ndarray] = []
+        seen = {}
+        for i, idx in enumerate(indices):
+            if float(idx) in seen:
+                continue
+            seen[float(idx)] = len(self.params)
+            self.params.append(float(idx))
+            self.rows.append(data[i])
+        order = np.argsort(np.array(self.params), kind="stable")
+        self.params = [self.params[i] for i in order]
+        self.rows = [self.rows[i] for i in order]
+        if n:
+            self.support = (float(indices.min()),
+                            float(indices.max()) + math.ldexp(1.0, -self.resolution))
+        else:
+            self.support = (0.0, 0.0)
+
+    def _rows_in(self, inf: float, sup: float) -> List[np.ndarray]:
+        return [r for p, r in zip(self.params, self.rows) if inf <= p < sup]
+
+    def _unit(self):
+        s = np.zeros(tensor_size(self.width, self.depth))
+        s[0] = 1.0
+        return s
+
+    def _piece_sig(self, k: int, n: int) -> Optional[np.ndarray]:
+        r = self.resolution
+        di = _DI(k, n)
+        if not self._rows_in(di.inf(), di.sup()):
+            return None
+        if n == r:
+            rows = self._rows_in(di.inf(), di.sup())
+            return signature(np.array(rows), self.width, self.depth)
+        if n > r:
+            leaf = _DI.from_dyadic(k, n, r)
+            if leaf.inf() == di.inf():
+                return self._piece_sig(leaf.k, leaf.n)
+            return None
+        lhs = self._piece_sig(2 * k, n + 1)
+        rhs = self._piece_sig(2 * k + 1, n + 1)
+        if lhs is None:
+            return rhs
+        if rhs is None:
+            return lhs
+        return ft_mul(lhs, rhs, self.width, self.depth)
+
+    def query_resolution(self, inf: float, sup: float) -> Optional[Tuple[float, float, int]]:
+        lo, hi = self.support
+        if sup < lo or inf > hi:
+            return None
+        a, b = max(inf, lo), min(sup, hi)
+        if b - a == 0.0:
+            return None
+        return a, b, max(self.resolution, param_to_resolution(b - a) + 2)
+
+    def signature(self, inf: float, sup: float, resolution: Optional[int] = None) -> np.ndarray:
+        q = self.query_resolution(inf, sup)
+        if q is None:
+            return self._unit()
+        a, b, res = q
+        if resolution is not None:
+            res = resolution
+        result = self._unit()
+        for k, n in to_dyadic_intervals(a, b, res):
+            s = self._piece_sig(k, n)
+            if s is not None:
+                result = ft_mul(result, s, self.width, self.depth)
+        return result
+
+    def log_signature(self, inf: float, sup: float, resolution: Optional[int] = None) -> np.ndarray:
+        s = self.signature(inf, sup, resolution)
+        return tensor_to_lie(ft_log(s, self.width, self.depth), self.width, self.depth)

@@ -1,0 +1,21 @@
+"""roughpy_amd -- RoughPy's dense free-tensor hot path on AMD MI355X (gfx950).
+
+Modules:
+  compute  -- drop-in for roughpy.compute (ft_fma, ft_exp, ft_log, ...)
+  streams  -- LieIncrementStream and batched signatures / log-signatures
+  bases    -- TensorBasis, LieBasis (Hall basis), sparse l2t / t2l maps
+
+All compute runs in hand-written HIP kernels in _lib/libroughpy_amd.so; the
+package never computes on the CPU.
+"""
+from . import _capi
+from .bases import LieBasis, SparseMatrix, TensorBasis, check_lie_basis_data
+
+__version__ = "0.1.0"
+
+__all__ = ["TensorBasis", "LieBasis", "SparseMatrix", "check_lie_basis_data", "native_library"]
+
+
+def native_library():
+    """The loaded native library (raises ImportError if it was not built)."""
+    return _capi.lib()

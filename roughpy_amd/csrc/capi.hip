@@ -1,0 +1,280 @@
+// capi.hip -- the extern "C" boundary of libroughpy_amd.so.
+//
+// Thin: argument checks, error reporting (status code + thread-local
+// message, the C analogue of the TypeError/ValueError the reference
+// bindings raise in roughpy/compute/_src/py_ternary_array_fn.hpp:83-160 and
+// call_config.cpp:11-31), and dispatch to the kernels.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/roughpy_amd.h"
+#include "chen.hpp"
+#include "dense_ops.hpp"
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+namespace {
+thread_local std::string g_last_error;
+
+__global__ void widen_kernel(double* __restrict__ out, const float* __restrict__ in, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = double(in[i]);
+}
+
+__global__ void zero_kernel(double* __restrict__ out, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = 0.0;
+}
+
+uint32_t blocks_for(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return uint32_t(b);
+}
+}  // namespace
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int make_basis(const rpy_tensor_basis* tb, DevBasis& out)
+{
+    if (!tb) return fail(RPY_ERR_INVALID_ARGUMENT, "null basis");
+    if (tb->width < 1) return fail(RPY_ERR_BASIS, "width must be positive, got %d", tb->width);
+    if (tb->depth < 0 || tb->depth > kMaxDepth)
+        return fail(RPY_ERR_BASIS, "depth must be in [0, %d], got %d", kMaxDepth, tb->depth);
+    out.width = tb->width;
+    out.depth = tb->depth;
+    out.db[0] = 0;
+    for (int k = 0; k <= tb->depth; ++k) {
+        if (out.db[k] > (INT64_C(1) << 40) / tb->width)
+            return fail(RPY_ERR_BASIS, "tensor algebra (width %d, depth %d) is too large",
+                        tb->width, tb->depth);
+        out.db[k + 1] = 1 + int64_t(tb->width) * out.db[k];
+    }
+    // the kernels index levels with 32-bit offsets
+    if (out.level_size(tb->depth) >= (INT64_C(1) << 31))
+        return fail(RPY_ERR_BASIS, "top level of (width %d, depth %d) exceeds 2^31 entries",
+                    tb->width, tb->depth);
+    if (tb->degree_begin) {
+        for (int k = 0; k <= tb->depth + 1; ++k)
+            if (tb->degree_begin[k] != out.db[k])
+                return fail(RPY_ERR_BASIS, "degree_begin[%d] = %lld does not match the dense layout",
+                            k, (long long)tb->degree_begin[k]);
+    }
+    return RPY_OK;
+}
+
+}  // namespace rpy_amd
+
+using namespace rpy_amd;
+
+extern "C" {
+
+const char* rpy_last_error(void) { return g_last_error.c_str(); }
+
+int32_t rpy_version(void) { return 1; }
+
+int32_t rpy_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rpy_release_workspace(void)
+{
+    // Temporaries are stream-ordered allocations returned at the end of
+    // each call; trim the device pool back to the system.
+    int dev = 0;
+    RPY_HIP_CHECK(hipGetDevice(&dev));
+    hipMemPool_t pool;
+    RPY_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+    RPY_HIP_CHECK(hipMemPoolTrimTo(pool, 0));
+    return RPY_OK;
+}
+
+int64_t rpy_tensor_size(int32_t width, int32_t depth) { return tensor_size_checked(width, depth); }
+
+int64_t rpy_lie_basis_size(int32_t width, int32_t depth)
+{
+    if (width < 1 || depth < 0) return -1;
+    return lie_size(width, depth);
+}
+
+int rpy_lie_basis_build(int32_t width, int32_t depth, int64_t* degree_begin, int64_t* data)
+{
+    if (width < 1 || depth < 0) return fail(RPY_ERR_BASIS, "invalid width/depth");
+    if (!degree_begin || !data) return fail(RPY_ERR_INVALID_ARGUMENT, "null output array");
+    return build_hall(width, depth, degree_begin, data);
+}
+
+int rpy_check_lie_basis_data(const rpy_lie_basis* basis)
+{
+    if (!basis || !basis->degree_begin || !basis->data)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null basis data");
+    return check_hall(basis);
+}
+
+int rpy_l2t_matrix(const rpy_lie_basis* basis, int64_t* nnz, int64_t* offsets, int64_t* indices,
+                   double* data)
+{
+    if (!basis || !basis->degree_begin || !basis->data || !nnz)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_hall(basis);
+    if (rc) return rc;
+    return build_l2t(basis, nnz, offsets, indices, data);
+}
+
+int rpy_t2l_matrix(const rpy_lie_basis* basis, int64_t* nnz, int64_t* offsets, int64_t* indices,
+                   double* data)
+{
+    if (!basis || !basis->degree_begin || !basis->data || !nnz)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_hall(basis);
+    if (rc) return rc;
+    return build_t2l(basis, nnz, offsets, indices, data);
+}
+
+int rpy_dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, rpy_degree_bounds ob, rpy_degree_bounds lb,
+                     rpy_degree_bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
+                     void* stream)
+{
+    return dense_ft_fma(out, lhs, rhs, dtype, basis, ob, lb, rb, batch, os, ls, rs, stream);
+}
+
+int rpy_dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
+                             const rpy_tensor_basis* basis, rpy_degree_bounds lb,
+                             rpy_degree_bounds rb, int64_t batch, int64_t ls, int64_t rs,
+                             void* stream)
+{
+    return dense_ft_inplace_mul(lhs, rhs, dtype, basis, lb, rb, batch, ls, rs, stream);
+}
+
+int rpy_dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype,
+                          const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t as,
+                          void* stream)
+{
+    return dense_ft_antipode(out, arg, dtype, basis, batch, os, as, stream);
+}
+
+int rpy_dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                                  const rpy_tensor_basis* basis, int64_t batch, int64_t os,
+                                  int64_t ps, int64_t as, void* stream)
+{
+    return dense_ft_adjoint_left_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+}
+
+int rpy_dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                     int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    return dense_ft_exp(out, arg, dtype, basis, batch, os, as, stream);
+}
+
+int rpy_dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
+                       const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
+                       int64_t es, void* stream)
+{
+    return dense_ft_fmexp(out, mul, expo, dtype, basis, batch, os, ms, es, stream);
+}
+
+int rpy_dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                     int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    return dense_ft_log(out, arg, dtype, basis, batch, os, as, stream);
+}
+
+int rpy_dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* matrix,
+                         rpy_dtype dtype, double scale, int64_t batch, int64_t os, int64_t as,
+                         void* stream)
+{
+    return dense_sparse_map(out, arg, matrix, dtype, scale, batch, os, as, stream);
+}
+
+int rpy_stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
+                               int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
+                               void* stream)
+{
+    return stream_signature_batch(out, increments, offsets, n_streams, width, depth, dtype,
+                                  stream);
+}
+
+int rpy_stream_log_signature_batch(double* out, const void* increments, const int64_t* offsets,
+                                   int64_t n_streams, int32_t width, int32_t depth,
+                                   rpy_dtype dtype, const rpy_sparse_matrix* t2l, void* stream)
+{
+    if (!t2l) return fail(RPY_ERR_INVALID_ARGUMENT, "null t2l matrix");
+    const int64_t tsize = tensor_size_checked(width, depth);
+    const int64_t lsize = (width >= 1 && depth >= 0) ? lie_size(width, depth) : -1;
+    if (tsize < 0 || lsize < 0) return fail(RPY_ERR_BASIS, "invalid width/depth");
+    if (t2l->rows != lsize || t2l->cols != tsize)
+        return fail(RPY_ERR_SHAPE, "t2l matrix must be %lld x %lld", (long long)lsize,
+                    (long long)tsize);
+    if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
+    if (n_streams == 0) return RPY_OK;
+    if (!out) return fail(RPY_ERR_INVALID_ARGUMENT, "null output");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const size_t tel = size_t(n_streams) * size_t(tsize);
+    const size_t elt = (dtype == RPY_F64) ? 8 : 4;
+    // sig (dtype) | sig widened to f64 (if needed) | log (f64)
+    char* ws = nullptr;
+    const size_t bytes = tel * elt + (dtype == RPY_F64 ? 0 : tel * 8) + tel * 8;
+    RPY_HIP_CHECK(hipMallocAsync((void**)&ws, bytes, st));
+    void* sig = ws;
+    double* sig64 = (dtype == RPY_F64) ? (double*)ws : (double*)(ws + tel * elt);
+    double* lg = (double*)(ws + tel * elt + (dtype == RPY_F64 ? 0 : tel * 8));
+    int rc = stream_signature_batch(sig, increments, offsets, n_streams, width, depth, dtype,
+                                    stream);
+    if (!rc && dtype != RPY_F64) {
+        hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0, st, sig64,
+                           (const float*)sig, int64_t(tel));
+        if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
+    }
+    rpy_tensor_basis tb{width, depth, nullptr};
+    if (!rc) rc = dense_ft_log(lg, sig64, RPY_F64, &tb, n_streams, tsize, tsize, stream);
+    if (!rc) {
+        hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(n_streams * lsize)), dim3(256), 0, st,
+                           out, n_streams * lsize);
+        if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "zero launch failed");
+    }
+    if (!rc) rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize, stream);
+    hipError_t ferr = hipFreeAsync(ws, st);
+    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    return rc;
+}
+
+// Test hook (not part of the compute surface): runs the lane decomposition
+// of the signature kernel on the host for one float64 stream.
+int rpy_debug_signature_host_emulation(double* out, const double* increments, int64_t length,
+                                       int32_t width, int32_t depth, int32_t prefix)
+{
+    if (!out || (!increments && length > 0)) return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
+    return signature_host_emulation(out, increments, length, width, depth, prefix);
+}
+
+// Prefix length the specialised signature kernel uses, 0 = generic kernel.
+int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth, int32_t is_f64)
+{
+    return chen_prefix_length(width, depth, is_f64);
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+// chen.hip -- batched Chen signatures of uneven piecewise-linear streams.
+//
+// One persistent launch: streams are sorted by length (longest first), every
+// stream is split across W^P lanes (chen_core.hpp), and each wave64 pulls the
+// next 64 (stream, prefix) items from a global work counter until the queue
+// is empty.  Each lane keeps its share of the running signature in
+// registers for the whole stream and writes it once at the end, so the only
+// HBM traffic is the increments in and the signatures out.
+//
+// Reference: Context::compute_signature, algebra/src/lite_context.h:545-563
+// (result = 1; result.fmexp_inplace(l2t(row)) per increment), with the fmexp
+// Horner of roughpy_compute/dense/intermediate/free_tensor_fmexp.hpp:15-40.
+
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "chen.hpp"
+#include "chen_core.hpp"
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+namespace {
+
+constexpr int kWave = 64;
+
+// keys[i] = length of stream i, vals[i] = i; also clears the queue counter.
+__global__ void prep_lengths_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                                    uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                    uint32_t* __restrict__ counter)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) *counter = 0u;
+    if (i < n) {
+        const int64_t len = offsets[i + 1] - offsets[i];
+        keys[i] = uint32_t(len < 0 ? 0 : (len > 0xffffffffLL ? 0xffffffffLL : len));
+        vals[i] = int32_t(i);
+    }
+}
+
+template <typename T, int W, int D, int P>
+__global__ __launch_bounds__(kWave) void chen_persistent_kernel(
+        T* __restrict__ out, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
+        const int32_t* __restrict__ order, int64_t n_items, int64_t tsize, DevBasis b,
+        uint32_t* __restrict__ counter)
+{
+    using Sh = chen::Shape<W, D, P>;
+    constexpr int NP = Sh::NP;
+    const int lane = threadIdx.x;
+    while (true) {
+        uint32_t chunk = 0;
+        if (lane == 0) chunk = atomicAdd(counter, 1u);
+        chunk = __shfl(chunk, 0);
+        const int64_t item = int64_t(chunk) * kWave + lane;
+        if (int64_t(chunk) * kWave >= n_items) break;
+        const bool active = item < n_items;
+        const int64_t si = active ? item / NP : 0;
+        const int q = active ? int(item % NP) : 0;
+        const int s = active ? order[si] : 0;
+        const int64_t row0 = active ? offsets[s] : 0;
+        const int64_t len = active ? offsets[s + 1] - row0 : 0;
+
+        int u[P];
+        {
+            int t = q;
+            for (int j = P - 1; j >= 0; --j) {
+                u[j] = t % W;
+                t /= W;
+            }
+        }
+
+        chen::Lane<T, W, D, P> st;
+        st.init_identity();
+
+        const T* rp = incr + row0 * W;
+        for (int64_t t = 0; t < len; ++t, rp += W) {
+            T x[W];
+            T xu[P];
+#pragma unroll
+            for (int a = 0; a < W; ++a) x[a] = rp[a];
+#pragma unroll
+            for (int j = 0; j < P; ++j) xu[j] = rp[u[j]];
+            st.step(x, xu);
+        }
+
+        if (active) {
+            T* o = out + int64_t(s) * tsize;
+            // subtree levels: lane q owns words q*W^(k-P) .. (q+1)*W^(k-P)-1
+            chen::static_for<D - P + 1>([&](auto i) {
+                constexpr int k = P + int(decltype(i)::value);
+                constexpr int nk = int(ipow(W, k - P));
+                T* ok = o + b.db[k] + int64_t(q) * nk;
+                chen::static_for<nk>([&](auto v) { ok[v] = st.sub[Sh::off(k) + v]; });
+            });
+            // path levels: written by the first lane of each prefix group
+            chen::static_for<P>([&](auto i) {
+                constexpr int k = int(decltype(i)::value);
+                constexpr int grp = int(ipow(W, P - k));
+                if (q % grp == 0) o[b.db[k] + q / grp] = st.path[k];
+            });
+        }
+    }
+}
+
+// Fallback for shapes without a specialised kernel: one workgroup per
+// stream, the running signature in the output buffer, per-coefficient Horner
+// with a barrier between levels.
+template <typename T>
+__global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
+                                                          const T* __restrict__ incr,
+                                                          const int64_t* __restrict__ offsets,
+                                                          int64_t n_streams, int64_t tsize,
+                                                          DevBasis b)
+{
+    const int W = b.width, D = b.depth;
+    for (int64_t s = blockIdx.x; s < n_streams; s += gridDim.x) {
+        T* S = out + s * tsize;
+        for (int64_t i = threadIdx.x; i < tsize; i += blockDim.x) S[i] = (i == 0) ? T(1) : T(0);
+        __syncthreads();
+        for (int64_t r = offsets[s]; r < offsets[s + 1]; ++r) {
+            const T* x = incr + r * W;
+            for (int m = D; m >= 1; --m) {
+                const int64_t n = b.level_size(m);
+                for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) {
+                    // letters of the word, most significant first
+                    T p = S[0];
+                    int64_t rest = idx, scale = n / W;
+                    for (int K = 1; K <= m; ++K) {
+                        const int64_t pre = idx / scale;  // prefix of length K
+                        const int letter = int(rest / scale);
+                        rest -= int64_t(letter) * scale;
+                        p = __builtin_fma(p, x[letter] / T(m - K + 1), S[b.db[K] + pre]);
+                        scale = (scale >= W) ? scale / W : 1;
+                    }
+                    // only this thread reads S_m[idx]; lower levels stay old
+                    // until the barrier below.
+                    S[b.db[m] + idx] = p;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+using Launcher = int (*)(void* out, const void* incr, const int64_t* offsets,
+                         const int32_t* order, int64_t n_streams, const DevBasis& b,
+                         uint32_t* counter, hipStream_t st);
+
+template <typename T, int W, int D, int P>
+int launch_persistent(void* out, const void* incr, const int64_t* offsets, const int32_t* order,
+                      int64_t n_streams, const DevBasis& b, uint32_t* counter, hipStream_t st)
+{
+    constexpr int NP = chen::Shape<W, D, P>::NP;
+    const int64_t n_items = n_streams * NP;
+    int dev = 0, cus = 0, per_cu = 0;
+    RPY_HIP_CHECK(hipGetDevice(&dev));
+    RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, chen_persistent_kernel<T, W, D, P>, kWave, 0));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t chunks = (n_items + kWave - 1) / kWave;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(chunks, int64_t(cus) * per_cu));
+    hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P>), dim3(uint32_t(grid)), dim3(kWave), 0,
+                       st, (T*)out, (const T*)incr, offsets, order, n_items, b.size(), b,
+                       counter);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+// Prefix length per shape: the smallest P whose per-lane state fits the
+// register budget (about 160 VGPRs of state for float, 128 for double).
+constexpr int choose_p(int W, int D, int budget)
+{
+    int p = 1;
+    while (p < D) {
+        int sub = 0;
+        for (int j = p; j <= D; ++j) sub += int(ipow(W, j - p));
+        if (sub + p <= budget) break;
+        ++p;
+    }
+    return p;
+}
+
+constexpr int kBudgetF32 = 160;  // state values per lane (1 VGPR each)
+constexpr int kBudgetF64 = 64;   // state values per lane (2 VGPRs each)
+
+struct Entry {
+    int w, d, p32, p64;
+    Launcher f32, f64;
+};
+
+template <int W, int D>
+Entry make_entry()
+{
+    constexpr int P32 = choose_p(W, D, kBudgetF32);
+    constexpr int P64 = choose_p(W, D, kBudgetF64);
+    return Entry{W, D, P32, P64, &launch_persistent<float, W, D, P32>,
+                 &launch_persistent<double, W, D, P64>};
+}
+
+const std::vector<Entry>& table()
+{
+    static const std::vector<Entry> t = {
+            make_entry<2, 1>(), make_entry<2, 2>(), make_entry<2, 3>(), make_entry<2, 4>(),
+            make_entry<2, 5>(), make_entry<2, 6>(), make_entry<2, 7>(), make_entry<2, 8>(),
+            make_entry<3, 1>(), make_entry<3, 2>(), make_entry<3, 3>(), make_entry<3, 4>(),
+            make_entry<3, 5>(), make_entry<3, 6>(), make_entry<4, 1>(), make_entry<4, 2>(),
+            make_entry<4, 3>(), make_entry<4, 4>(), make_entry<4, 5>(), make_entry<4, 6>(),
+            make_entry<5, 1>(), make_entry<5, 2>(), make_entry<5, 3>(), make_entry<5, 4>(),
+            make_entry<5, 5>(), make_entry<6, 1>(), make_entry<6, 2>(), make_entry<6, 3>(),
+            make_entry<6, 4>(), make_entry<6, 5>(), make_entry<7, 1>(), make_entry<7, 2>(),
+            make_entry<7, 3>(), make_entry<7, 4>(), make_entry<8, 1>(), make_entry<8, 2>(),
+            make_entry<8, 3>(), make_entry<8, 4>(),
+    };
+    return t;
+}
+
+const Entry* find_entry(int w, int d)
+{
+    for (const auto& e : table())
+        if (e.w == w && e.d == d) return &e;
+    return nullptr;
+}
+
+}  // namespace
+
+int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64)
+{
+    const Entry* e = find_entry(width, depth);
+    if (!e) return 0;
+    return dtype_is_f64 ? e->p64 : e->p32;
+}
+
+int stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
+                           int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
+                           void* stream)
+{
+    if (dtype == RPY_F32_COMPENSATED)
+        return fail(RPY_ERR_UNSUPPORTED, "compensated float32 signatures are not available yet");
+    if (dtype != RPY_F32 && dtype != RPY_F64)
+        return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    rpy_tensor_basis tb{width, depth, nullptr};
+    DevBasis b;
+    int rc = make_basis(&tb, b);
+    if (rc) return rc;
+    if (depth < 1) return fail(RPY_ERR_INVALID_ARGUMENT, "signatures need depth >= 1");
+    if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
+    if (n_streams == 0) return RPY_OK;
+    if (n_streams > INT32_MAX) return fail(RPY_ERR_SHAPE, "too many streams");
+    if (!out || !offsets || !increments)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+    const Entry* e = find_entry(width, depth);
+    if (!e) {
+        const int grid = int(std::min<int64_t>(n_streams, 4096));
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL(chen_generic_kernel<double>, dim3(grid), dim3(256), 0, st,
+                               (double*)out, (const double*)increments, offsets, n_streams,
+                               b.size(), b);
+        else
+            hipLaunchKernelGGL(chen_generic_kernel<float>, dim3(grid), dim3(256), 0, st,
+                               (float*)out, (const float*)increments, offsets, n_streams,
+                               b.size(), b);
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
+
+    // workspace: keys in/out, values in/out, counter, radix-sort scratch
+    const size_t n = size_t(n_streams);
+    size_t sort_bytes = 0;
+    RPY_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, (uint32_t*)nullptr,
+                                                 (uint32_t*)nullptr, (int32_t*)nullptr,
+                                                 (int32_t*)nullptr, n, 0, 32, st));
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t k_bytes = align(n * sizeof(uint32_t));
+    const size_t v_bytes = align(n * sizeof(int32_t));
+    const size_t total = 2 * k_bytes + 2 * v_bytes + 256 + align(sort_bytes);
+    char* ws = nullptr;
+    RPY_HIP_CHECK(hipMallocAsync((void**)&ws, total, st));
+    uint32_t* keys_in = (uint32_t*)ws;
+    uint32_t* keys_out = (uint32_t*)(ws + k_bytes);
+    int32_t* vals_in = (int32_t*)(ws + 2 * k_bytes);
+    int32_t* vals_out = (int32_t*)(ws + 2 * k_bytes + v_bytes);
+    uint32_t* counter = (uint32_t*)(ws + 2 * k_bytes + 2 * v_bytes);
+    void* sort_tmp = ws + 2 * k_bytes + 2 * v_bytes + 256;
+
+    const int pb = 256;
+    hipLaunchKernelGGL(prep_lengths_kernel, dim3(uint32_t((n + pb - 1) / pb)), dim3(pb), 0, st,
+                       offsets, n_streams, keys_in, vals_in, counter);
+    rc = RPY_OK;
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "length prep launch failed: %s", hipGetErrorString(err));
+    if (!rc) {
+        err = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys_in, keys_out, vals_in,
+                                             vals_out, n, 0, 32, st);
+        if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
+    }
+    if (!rc) {
+        Launcher f = (dtype == RPY_F64) ? e->f64 : e->f32;
+        rc = f(out, increments, offsets, vals_out, n_streams, b, counter, st);
+    }
+    hipError_t ferr = hipFreeAsync(ws, st);
+    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ferr));
+    return rc;
+}
+
+// Runs the lane algorithm of the specialised kernel on the host for one
+// stream (double precision) -- used by CPU tests to check the decomposition
+// and the output index map without a GPU.
+template <int W, int D, int P>
+static void emulate(double* out, const double* incr, int64_t len, const DevBasis& b)
+{
+    using Sh = chen::Shape<W, D, P>;
+    for (int q = 0; q < Sh::NP; ++q) {
+        int u[P];
+        int t = q;
+        for (int j = P - 1; j >= 0; --j) {
+            u[j] = t % W;
+            t /= W;
+        }
+        chen::Lane<double, W, D, P> st;
+        st.init_identity();
+        for (int64_t r = 0; r < len; ++r) {
+            double x[W], xu[P];
+            for (int a = 0; a < W; ++a) x[a] = incr[r * W + a];
+            for (int j = 0; j < P; ++j) xu[j] = incr[r * W + u[j]];
+            st.step(x, xu);
+        }
+        for (int k = P; k <= D; ++k) {
+            const int nk = int(ipow(W, k - P));
+            for (int v = 0; v < nk; ++v) out[b.db[k] + int64_t(q) * nk + v] = st.sub[Sh::off(k) + v];
+        }
+        for (int k = 0; k < P; ++k) {
+            const int grp = int(ipow(W, P - k));
+            if (q % grp == 0) out[b.db[k] + q / grp] = st.path[k];
+        }
+    }
+}
+
+int signature_host_emulation(double* out, const double* incr, int64_t len, int32_t width,
+                             int32_t depth, int32_t prefix)
+{
+    rpy_tensor_basis tb{width, depth, nullptr};
+    DevBasis b;
+    int rc = make_basis(&tb, b);
+    if (rc) return rc;
+#define RPY_EMU(W, D, P) \
+    if (width == W && depth == D && prefix == P) { emulate<W, D, P>(out, incr, len, b); return RPY_OK; }
+    RPY_EMU(2, 3, 1) RPY_EMU(2, 3, 2) RPY_EMU(2, 3, 3)
+    RPY_EMU(3, 4, 1) RPY_EMU(3, 4, 2) RPY_EMU(3, 4, 3)
+    RPY_EMU(5, 5, 2) RPY_EMU(5, 5, 3)
+    RPY_EMU(4, 6, 3) RPY_EMU(4, 6, 4)
+    RPY_EMU(8, 4, 2) RPY_EMU(8, 4, 3)
+    RPY_EMU(6, 5, 3)
+#undef RPY_EMU
+    return fail(RPY_ERR_UNSUPPORTED, "no host emulation for this (width, depth, prefix)");
+}
+
+}  // namespace rpy_amd

@@ -1,0 +1,620 @@
+// dense_ops.hip -- batched dense free-tensor operations for gfx950.
+//
+// One workgroup (256 lanes = 4 wave64) owns one batch element at a time
+// (grid-stride over the batch).  The lower levels of the operands, which
+// every output coefficient re-reads, are staged in LDS; the top level --
+// read once per coefficient -- streams straight from HBM, so an fma moves
+// 4 * size * sizeof(T) bytes per element and no more.
+//
+// Reference kernels mirrored (same per-coefficient term order where it
+// matters for rounding):
+//   ft_fma          roughpy_compute/dense/basic/free_tensor_fma.hpp:13-53
+//   ft_inplace_mul  roughpy_compute/dense/basic/free_tensor_inplace_mul.hpp:13-87
+//   ft_antipode     roughpy_compute/dense/basic/free_tensor_antipode.hpp:45-100
+//   ft_adj_lmul     roughpy_compute/dense/basic/free_tensor_adjoint_left_mul.hpp:10-77
+//   ft_exp          roughpy_compute/dense/intermediate/free_tensor_exp.hpp:13-37
+//   ft_fmexp        roughpy_compute/dense/intermediate/free_tensor_fmexp.hpp:15-40
+//   ft_log          roughpy_compute/dense/intermediate/free_tensor_log.hpp:12-40
+//   sparse map      roughpy_compute/dense/basic/apply_sparse_linear_map.hpp:16-55
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dense_ops.hpp"
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+struct Bounds {
+    int32_t lo, hi;
+};
+
+// Exact unsigned division by a runtime divisor d >= 1 for n < 2^31
+// (Granlund-Montgomery round-up method: q = (umulhi(m, n) + n) >> s).
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+FastDiv make_fastdiv(uint32_t d)
+{
+    uint32_t s = 0;
+    while ((uint64_t(1) << s) < d) ++s;
+    uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1;
+    return FastDiv{d, uint32_t(m), s};
+}
+
+RPY_FI uint32_t fdiv(uint32_t n, const FastDiv& f)
+{
+    return (__umulhi(f.m, n) + n) >> f.s;
+}
+
+// Per-basis powers of the width as fast divisors: pw[k] divides by w^k.
+struct Pow {
+    FastDiv pw[kMaxDepth + 1];
+};
+
+Pow make_pow(int w, int D)
+{
+    Pow p{};
+    uint64_t v = 1;
+    for (int k = 0; k <= D && k <= kMaxDepth; ++k) {
+        p.pw[k] = make_fastdiv(uint32_t(v));
+        v *= uint64_t(w);
+    }
+    return p;
+}
+
+int grid_for(int64_t batch)
+{
+    // Enough workgroups to fill 256 CUs several times over; the kernels
+    // grid-stride over the rest of the batch.
+    return int(std::min<int64_t>(batch, 8192));
+}
+
+// ---------------------------------------------------------------------------
+// out += lhs (x) rhs
+// ---------------------------------------------------------------------------
+template <typename T, bool Stage>
+__global__ __launch_bounds__(kBlock) void ft_fma_kernel(
+        T* __restrict__ out, const T* __restrict__ lhs, const T* __restrict__ rhs,
+        DevBasis b, Pow pw, Bounds ob, Bounds lb, Bounds rb, int64_t batch, int64_t os,
+        int64_t ls, int64_t rs, int32_t top)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sl = reinterpret_cast<T*>(smem);
+    T* sr = sl + b.db[top];
+    const int omin = ob.lo > 1 ? ob.lo : 1;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        T* o = out + e * os;
+        const T* l = lhs + e * ls;
+        const T* r = rhs + e * rs;
+        const T* L = l;
+        const T* R = r;
+        if (Stage) {
+            // levels below `top` are re-read by many outputs: stage them.
+            for (int64_t i = threadIdx.x; i < b.db[top]; i += kBlock) {
+                sl[i] = l[i];
+                sr[i] = r[i];
+            }
+            __syncthreads();
+            L = sl;
+            R = sr;
+        }
+        for (int od = ob.hi; od >= omin; --od) {
+            const int lmax = min(lb.hi, od - rb.lo);
+            const int lmin = max(lb.lo, od - rb.hi);
+            if (lmax < lmin) continue;
+            const uint32_t n = uint32_t(b.level_size(od));
+            for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+                T acc = o[b.db[od] + idx];
+                for (int ld = lmax; ld >= lmin; --ld) {
+                    const int rd = od - ld;
+                    const uint32_t i = fdiv(idx, pw.pw[rd]);
+                    const uint32_t j = idx - i * pw.pw[rd].d;
+                    const T lv = (ld < top) ? L[b.db[ld] + i] : l[b.db[ld] + i];
+                    const T rv = (rd < top) ? R[b.db[rd] + j] : r[b.db[rd] + j];
+                    acc += lv * rv;
+                }
+                o[b.db[od] + idx] = acc;
+            }
+        }
+        if (threadIdx.x == 0 && ob.lo == 0 && lb.lo == 0 && rb.lo == 0) o[0] += l[0] * r[0];
+        if (Stage) __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// lhs <- lhs (x) rhs   (old lower levels of lhs are staged before writing)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void ft_inplace_mul_kernel(
+        T* __restrict__ lhs, const T* __restrict__ rhs, DevBasis b, Pow pw, Bounds lb,
+        Bounds rb, int64_t batch, int64_t ls, int64_t rs, int32_t top)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    // Levels [0, max(top, 1)) are staged, so the unit is always in LDS.
+    const int64_t nstage = b.db[top > 0 ? top : 1];
+    T* sl = reinterpret_cast<T*>(smem);
+    T* sr = sl + nstage;
+    const int lo = lb.lo > 1 ? lb.lo : 1;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        T* l = lhs + e * ls;
+        const T* r = rhs + e * rs;
+        for (int64_t i = threadIdx.x; i < nstage; i += kBlock) {
+            sl[i] = l[i];
+            sr[i] = r[i];
+        }
+        __syncthreads();
+        const T r0 = r[0];
+        for (int od = lb.hi; od >= lo; --od) {
+            const int lmin = max(lb.lo, od - rb.hi);
+            const int lmax = od - max(1, rb.lo);
+            const uint32_t n = uint32_t(b.level_size(od));
+            for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+                const T old = (od < top) ? sl[b.db[od] + idx] : l[b.db[od] + idx];
+                T acc = (rb.lo == 0) ? old * r0 : T(0);
+                for (int ld = lmax; ld >= lmin; --ld) {
+                    const int rd = od - ld;
+                    const uint32_t i = fdiv(idx, pw.pw[rd]);
+                    const uint32_t j = idx - i * pw.pw[rd].d;
+                    const T rv = (rd < top) ? sr[b.db[rd] + j] : r[b.db[rd] + j];
+                    acc += sl[b.db[ld] + i] * rv;
+                }
+                l[b.db[od] + idx] = acc;
+            }
+        }
+        if (threadIdx.x == 0) {
+            if (lb.lo == 0 && rb.lo == 0)
+                l[0] = sl[0] * r0;
+            else if (lb.lo == 0)
+                l[0] = T(0);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Horner forms of exp / fmexp / log on one element held by one workgroup.
+// `a` is the working tensor (LDS or the output buffer), `x` the argument.
+// ---------------------------------------------------------------------------
+
+// a.trunc(ml) <- (a.trunc(ml) (x) x.trunc(ml, min 1)) * f, top-down with a
+// barrier per level; afterwards a[0] = 0 (free_tensor_inplace_mul.hpp:21-86
+// with an implicitly zero unit on the right).
+template <typename T>
+RPY_FI void wg_mul_by_nounit(T* a, const T* x, const DevBasis& b, const Pow& pw, int ml, T f)
+{
+    for (int m = ml; m >= 1; --m) {
+        const uint32_t n = uint32_t(b.level_size(m));
+        for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+            T acc = T(0);
+            for (int ld = m - 1; ld >= 0; --ld) {
+                const int rd = m - ld;
+                const uint32_t i = fdiv(idx, pw.pw[rd]);
+                const uint32_t j = idx - i * pw.pw[rd].d;
+                acc += (a[b.db[ld] + i] * x[b.db[rd] + j]) * f;
+            }
+            a[b.db[m] + idx] = acc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[0] = T(0);
+    __syncthreads();
+}
+
+enum class Horner { Exp, FmExp, Log };
+
+template <typename T, Horner H, bool InLds>
+__global__ __launch_bounds__(kBlock) void horner_kernel(
+        T* __restrict__ out, const T* __restrict__ mul, const T* __restrict__ arg, DevBasis b,
+        Pow pw, int64_t batch, int64_t os, int64_t ms, int64_t as)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = b.depth;
+    const int64_t n = b.size();
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        T* o = out + e * os;
+        const T* x = arg + e * as;
+        const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
+        T* a = InLds ? reinterpret_cast<T*>(smem) : o;
+        const T* xs = x;
+        if (InLds) {
+            T* sx = reinterpret_cast<T*>(smem) + n;
+            for (int64_t i = threadIdx.x; i < n; i += kBlock) sx[i] = x[i];
+            xs = sx;
+        }
+        for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+            if (H == Horner::FmExp)
+                a[i] = m[i];
+            else
+                a[i] = (H == Horner::Exp && i == 0) ? T(1) : T(0);
+        }
+        __syncthreads();
+        for (int deg = D; deg > 0; --deg) {
+            const int ml = D - deg + 1;
+            if (H == Horner::Log) {
+                if (threadIdx.x == 0) {
+                    const T inv = T(1) / T(deg);
+                    a[0] = (deg % 2 == 0) ? a[0] - inv : a[0] + inv;
+                }
+                __syncthreads();
+                wg_mul_by_nounit(a, xs, b, pw, ml, T(1));
+            } else {
+                wg_mul_by_nounit(a, xs, b, pw, ml, T(1) / T(deg));
+                if (H == Horner::Exp) {
+                    if (threadIdx.x == 0) a[0] += T(1);
+                } else {
+                    for (int64_t i = threadIdx.x; i < n; i += kBlock) a[i] += m[i];
+                }
+                __syncthreads();
+            }
+        }
+        if (InLds) {
+            for (int64_t i = threadIdx.x; i < n; i += kBlock) o[i] = a[i];
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// antipode: out[rev(w)] = (-1)^|w| arg[w]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void antipode_kernel(T* __restrict__ out,
+                                                         const T* __restrict__ arg, DevBasis b,
+                                                         Pow pw, int64_t batch, int64_t os,
+                                                         int64_t as)
+{
+    const int64_t n = b.size();
+    const int64_t total = batch * n;
+    const uint32_t w = uint32_t(b.width);
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total;
+         t += int64_t(gridDim.x) * kBlock) {
+        const int64_t e = t / n;
+        const int64_t g = t - e * n;
+        int d = 0;
+        while (g >= b.db[d + 1]) ++d;
+        uint32_t i = uint32_t(g - b.db[d]);
+        uint32_t rev = 0;
+        for (int k = 0; k < d; ++k) {
+            const uint32_t q = fdiv(i, pw.pw[1]);
+            rev = rev * w + (i - q * w);
+            i = q;
+        }
+        const T v = arg[e * as + g];
+        out[e * os + b.db[d] + rev] = (d & 1) ? -v : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// out_k[i] += sum_{|u| = p} op_p[u] arg_{k+p}[u i]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void adj_lmul_kernel(T* __restrict__ out,
+                                                         const T* __restrict__ op,
+                                                         const T* __restrict__ arg, DevBasis b,
+                                                         int64_t batch, int64_t os, int64_t ps,
+                                                         int64_t as)
+{
+    const int D = b.depth;
+    const int64_t n = b.size();
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        const T* p = op + e * ps;
+        const T* a = arg + e * as;
+        T* o = out + e * os;
+        for (int64_t g = threadIdx.x; g < n; g += kBlock) {
+            int od = 0;
+            while (g >= b.db[od + 1]) ++od;
+            const int64_t i = g - b.db[od];
+            const int64_t osz = b.level_size(od);
+            T acc = o[g];
+            // argument degree descending == op degree descending
+            for (int pd = D - od; pd >= 0; --pd) {
+                const int ad = od + pd;
+                const int64_t psz = b.level_size(pd);
+                for (int64_t u = 0; u < psz; ++u)
+                    acc += p[b.db[pd] + u] * a[b.db[ad] + u * osz + i];
+            }
+            o[g] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR sparse map: out[e, r] += scale * sum_p data[p] arg[e, idx[p]]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void csr_map_kernel(T* __restrict__ out,
+                                                        const T* __restrict__ arg,
+                                                        const int64_t* __restrict__ offs,
+                                                        const int64_t* __restrict__ idx,
+                                                        const T* __restrict__ val, int64_t rows,
+                                                        T scale, int64_t batch, int64_t os,
+                                                        int64_t as)
+{
+    const int64_t total = batch * rows;
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total;
+         t += int64_t(gridDim.x) * kBlock) {
+        const int64_t e = t / rows;
+        const int64_t r = t - e * rows;
+        const T* a = arg + e * as;
+        T acc = T(0);
+        for (int64_t p = offs[r]; p < offs[r + 1]; ++p) acc += (val[p] * a[idx[p]]) * scale;
+        out[e * os + r] += acc;
+    }
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_bounds(const DevBasis& b, rpy_degree_bounds in, Bounds& out, const char* name)
+{
+    out.lo = in.min_degree;
+    out.hi = in.max_degree < 0 ? b.depth : in.max_degree;
+    if (out.lo < 0 || out.hi > b.depth || out.lo > out.hi)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "invalid degree bounds for %s: [%d, %d] with depth %d",
+                    name, in.min_degree, in.max_degree, b.depth);
+    return RPY_OK;
+}
+
+int check_batch(int64_t batch, std::initializer_list<int64_t> strides, int64_t need)
+{
+    if (batch < 0) return fail(RPY_ERR_SHAPE, "negative batch size");
+    for (int64_t s : strides)
+        if (batch > 1 && s < need)
+            return fail(RPY_ERR_SHAPE, "batch stride %lld smaller than the tensor size %lld",
+                        (long long)s, (long long)need);
+    return RPY_OK;
+}
+
+template <typename T>
+int launch_fma(void* out, const void* lhs, const void* rhs, const DevBasis& b, Bounds ob,
+               Bounds lb, Bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
+               hipStream_t st)
+{
+    const Pow pw = make_pow(b.width, b.depth);
+    const int top = ob.hi;
+    const size_t lds = size_t(2) * size_t(b.db[top]) * sizeof(T);
+    dim3 grid(grid_for(batch)), block(kBlock);
+    if (lds <= kLdsBudget) {
+        hipLaunchKernelGGL((ft_fma_kernel<T, true>), grid, block, lds, st, (T*)out,
+                           (const T*)lhs, (const T*)rhs, b, pw, ob, lb, rb, batch, os, ls, rs,
+                           top);
+    } else {
+        hipLaunchKernelGGL((ft_fma_kernel<T, false>), grid, block, 0, st, (T*)out,
+                           (const T*)lhs, (const T*)rhs, b, pw, ob, lb, rb, batch, os, ls, rs,
+                           0);
+    }
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+template <typename T>
+int launch_inplace(void* lhs, const void* rhs, const DevBasis& b, Bounds lb, Bounds rb,
+                   int64_t batch, int64_t ls, int64_t rs, hipStream_t st)
+{
+    const Pow pw = make_pow(b.width, b.depth);
+    const int top = lb.hi;
+    const size_t lds = size_t(2) * size_t(b.db[top > 0 ? top : 1]) * sizeof(T);
+    if (lds > kLdsBudget)
+        return fail(RPY_ERR_UNSUPPORTED,
+                    "in-place multiplication needs %zu bytes of LDS (limit %zu)", lds,
+                    kLdsBudget);
+    dim3 grid(grid_for(batch)), block(kBlock);
+    hipLaunchKernelGGL((ft_inplace_mul_kernel<T>), grid, block, lds, st, (T*)lhs,
+                       (const T*)rhs, b, pw, lb, rb, batch, ls, rs, top);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+template <typename T, Horner H>
+int launch_horner(void* out, const void* mul, const void* arg, const DevBasis& b,
+                  int64_t batch, int64_t os, int64_t ms, int64_t as, hipStream_t st)
+{
+    const Pow pw = make_pow(b.width, b.depth);
+    const size_t lds = size_t(2) * size_t(b.size()) * sizeof(T);
+    dim3 grid(grid_for(batch)), block(kBlock);
+    if (lds <= kLdsBudget) {
+        hipLaunchKernelGGL((horner_kernel<T, H, true>), grid, block, lds, st, (T*)out,
+                           (const T*)mul, (const T*)arg, b, pw, batch, os, ms, as);
+    } else {
+        hipLaunchKernelGGL((horner_kernel<T, H, false>), grid, block, 0, st, (T*)out,
+                           (const T*)mul, (const T*)arg, b, pw, batch, os, ms, as);
+    }
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+int dtype_size(rpy_dtype dt)
+{
+    switch (dt) {
+        case RPY_F32: return 4;
+        case RPY_F64: return 8;
+        default: return 0;
+    }
+}
+
+int plain_dtype(rpy_dtype dt)
+{
+    if (dt != RPY_F32 && dt != RPY_F64)
+        return fail(RPY_ERR_DTYPE, "only float32 and float64 are supported for this operation");
+    return RPY_OK;
+}
+
+int64_t blocks_elementwise(int64_t total)
+{
+    return std::max<int64_t>(1, std::min<int64_t>((total + kBlock - 1) / kBlock, 65536));
+}
+
+}  // namespace
+
+int dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                 const rpy_tensor_basis* basis, rpy_degree_bounds obi, rpy_degree_bounds lbi,
+                 rpy_degree_bounds rbi, int64_t batch, int64_t os, int64_t ls, int64_t rs,
+                 void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    Bounds ob, lb, rb;
+    if ((rc = check_bounds(b, obi, ob, "out")) || (rc = check_bounds(b, lbi, lb, "lhs")) ||
+        (rc = check_bounds(b, rbi, rb, "rhs")))
+        return rc;
+    if ((rc = check_batch(batch, {os, ls, rs}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !lhs || !rhs) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    return dtype == RPY_F64
+                   ? launch_fma<double>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
+                                        as_stream(stream))
+                   : launch_fma<float>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
+                                       as_stream(stream));
+}
+
+int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
+                         const rpy_tensor_basis* basis, rpy_degree_bounds lbi,
+                         rpy_degree_bounds rbi, int64_t batch, int64_t ls, int64_t rs,
+                         void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    Bounds lb, rb;
+    if ((rc = check_bounds(b, lbi, lb, "lhs")) || (rc = check_bounds(b, rbi, rb, "rhs")))
+        return rc;
+    if ((rc = check_batch(batch, {ls, rs}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!lhs || !rhs) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    return dtype == RPY_F64 ? launch_inplace<double>(lhs, rhs, b, lb, rb, batch, ls, rs,
+                                                     as_stream(stream))
+                            : launch_inplace<float>(lhs, rhs, b, lb, rb, batch, ls, rs,
+                                                    as_stream(stream));
+}
+
+int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                      int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    const Pow pw = make_pow(b.width, 1);
+    dim3 grid(blocks_elementwise(batch * b.size())), block(kBlock);
+    if (dtype == RPY_F64)
+        hipLaunchKernelGGL(antipode_kernel<double>, grid, block, 0, as_stream(stream),
+                           (double*)out, (const double*)arg, b, pw, batch, os, as);
+    else
+        hipLaunchKernelGGL(antipode_kernel<float>, grid, block, 0, as_stream(stream),
+                           (float*)out, (const float*)arg, b, pw, batch, os, as);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                              const rpy_tensor_basis* basis, int64_t batch, int64_t os,
+                              int64_t ps, int64_t as, void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    if ((rc = check_batch(batch, {os, ps, as}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !op || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    dim3 grid(grid_for(batch)), block(kBlock);
+    if (dtype == RPY_F64)
+        hipLaunchKernelGGL(adj_lmul_kernel<double>, grid, block, 0, as_stream(stream),
+                           (double*)out, (const double*)op, (const double*)arg, b, batch, os, ps,
+                           as);
+    else
+        hipLaunchKernelGGL(adj_lmul_kernel<float>, grid, block, 0, as_stream(stream),
+                           (float*)out, (const float*)op, (const float*)arg, b, batch, os, ps,
+                           as);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+static int horner_entry(Horner h, void* out, const void* mul, const void* arg, rpy_dtype dtype,
+                        const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
+                        int64_t as, void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
+    if (h == Horner::FmExp && (rc = check_batch(batch, {ms}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !arg || (h == Horner::FmExp && !mul))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    hipStream_t st = as_stream(stream);
+    if (dtype == RPY_F64) {
+        switch (h) {
+            case Horner::Exp: return launch_horner<double, Horner::Exp>(out, mul, arg, b, batch, os, ms, as, st);
+            case Horner::FmExp: return launch_horner<double, Horner::FmExp>(out, mul, arg, b, batch, os, ms, as, st);
+            case Horner::Log: return launch_horner<double, Horner::Log>(out, mul, arg, b, batch, os, ms, as, st);
+        }
+    } else {
+        switch (h) {
+            case Horner::Exp: return launch_horner<float, Horner::Exp>(out, mul, arg, b, batch, os, ms, as, st);
+            case Horner::FmExp: return launch_horner<float, Horner::FmExp>(out, mul, arg, b, batch, os, ms, as, st);
+            case Horner::Log: return launch_horner<float, Horner::Log>(out, mul, arg, b, batch, os, ms, as, st);
+        }
+    }
+    return fail(RPY_ERR_INVALID_ARGUMENT, "bad Horner kind");
+}
+
+int dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                 int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    return horner_entry(Horner::Exp, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+}
+
+int dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
+                   const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
+                   int64_t es, void* stream)
+{
+    return horner_entry(Horner::FmExp, out, mul, expo, dtype, basis, batch, os, ms, es, stream);
+}
+
+int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                 int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    return horner_entry(Horner::Log, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+}
+
+int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy_dtype dtype,
+                     double scale, int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    int rc;
+    if ((rc = plain_dtype(dtype))) return rc;
+    if (!m) return fail(RPY_ERR_INVALID_ARGUMENT, "null matrix");
+    if (!m->compressed_rows)
+        return fail(RPY_ERR_UNSUPPORTED,
+                    "the device sparse map takes CSR matrices; convert CSC on the host");
+    if (m->rows < 0 || m->cols < 0 || m->nnz < 0)
+        return fail(RPY_ERR_SHAPE, "negative matrix dimensions");
+    if (batch < 0) return fail(RPY_ERR_SHAPE, "negative batch size");
+    if (batch > 1 && (os < m->rows || as < m->cols))
+        return fail(RPY_ERR_SHAPE, "batch strides smaller than the matrix dimensions");
+    if (batch == 0 || m->rows == 0) return RPY_OK;
+    if (!out || !arg || !m->offsets || (m->nnz > 0 && (!m->indices || !m->data)))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    dim3 grid(blocks_elementwise(batch * m->rows)), block(kBlock);
+    if (dtype == RPY_F64)
+        hipLaunchKernelGGL(csr_map_kernel<double>, grid, block, 0, as_stream(stream),
+                           (double*)out, (const double*)arg, m->offsets, m->indices,
+                           (const double*)m->data, m->rows, scale, batch, os, as);
+    else
+        hipLaunchKernelGGL(csr_map_kernel<float>, grid, block, 0, as_stream(stream),
+                           (float*)out, (const float*)arg, m->offsets, m->indices,
+                           (const float*)m->data, m->rows, float(scale), batch, os, as);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+}  // namespace rpy_amd

@@ -1,0 +1,33 @@
+// dense_ops.hpp -- entry points of dense_ops.hip.
+#pragma once
+
+#include <cstdint>
+
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+int dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                 const rpy_tensor_basis* basis, rpy_degree_bounds ob, rpy_degree_bounds lb,
+                 rpy_degree_bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
+                 void* stream);
+int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
+                         const rpy_tensor_basis* basis, rpy_degree_bounds lb,
+                         rpy_degree_bounds rb, int64_t batch, int64_t ls, int64_t rs,
+                         void* stream);
+int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                      int64_t batch, int64_t os, int64_t as, void* stream);
+int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                              const rpy_tensor_basis* basis, int64_t batch, int64_t os,
+                              int64_t ps, int64_t as, void* stream);
+int dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                 int64_t batch, int64_t os, int64_t as, void* stream);
+int dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
+                   const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
+                   int64_t es, void* stream);
+int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
+                 int64_t batch, int64_t os, int64_t as, void* stream);
+int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy_dtype dtype,
+                     double scale, int64_t batch, int64_t os, int64_t as, void* stream);
+
+}  // namespace rpy_amd

@@ -1,0 +1,200 @@
+// streams_host.cpp -- host logic of stream queries: dyadic dissection of a
+// clopen interval and the selection of the increment rows a query covers.
+//
+// Mirrors (integer dyadic arithmetic, same results):
+//   DyadicInterval ops        intervals/src/dyadic_interval.cpp:47-183
+//   to_dyadic_intervals       intervals/src/dyadic_interval.cpp:191-228
+//   param_to_resolution       streams/include/roughpy/streams/stream_base.h:73-83
+//   Stream::refine_interval   streams/src/stream.cpp:37-71
+//   DyadicCachingLayer query  streams/src/dyadic_caching_layer.cpp:86-130
+//   leaf rows [inf, sup)      streams/src/lie_increment_stream.cpp:174-199
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+namespace {
+
+// Clopen dyadic interval [k 2^-n, (k+1) 2^-n).
+struct DI {
+    int64_t k;
+    int32_t n;
+};
+
+int64_t floor_div_pow2(int64_t k, int32_t s)
+{
+    // floor(k / 2^s) for s >= 0 (arithmetic shift floors for negatives)
+    return s >= 63 ? (k < 0 ? -1 : 0) : (k >> s);
+}
+
+DI rebase(DI d, int32_t resolution)
+{
+    if (resolution >= d.n) return DI{d.k * (int64_t(1) << (resolution - d.n)), resolution};
+    return DI{floor_div_pow2(d.k, d.n - resolution), resolution};
+}
+
+DI from_param(double v, int32_t resolution)
+{
+    return DI{int64_t(std::floor(std::ldexp(v, resolution))), resolution};
+}
+
+DI expand(DI d, int32_t arg = 1) { return rebase(d, d.n - arg); }
+
+bool aligned(DI d)
+{
+    const DI parent = rebase(d, d.n - 1);
+    const DI back = rebase(parent, d.n);
+    return back.k == d.k;
+}
+
+bool contains(DI a, DI other)
+{
+    if (other.n < a.n) return false;
+    const int32_t s = other.n - a.n;
+    const int64_t p = int64_t(1) << s;
+    return a.k * p == floor_div_pow2(other.k, s) * p;
+}
+
+DI shrink_contained(DI d) { return DI{d.k * 2, d.n + 1}; }
+DI flip(DI d) { return DI{(d.k % 2 == 0) ? d.k + 1 : d.k - 1, d.n}; }
+DI shrink_omitted(DI d) { return flip(shrink_contained(d)); }
+
+}  // namespace
+
+int32_t param_to_resolution(double arg)
+{
+    int e = 0;
+    std::frexp(arg, &e);
+    return -std::min(0, e - 1);
+}
+
+// Pieces of [inf, sup) at tolerance `tol`, left to right.
+int dyadic_dissection(double inf, double sup, int32_t tol, std::vector<DI>& out)
+{
+    out.clear();
+    if (!(std::isfinite(inf) && std::isfinite(sup)))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "interval end points must be finite");
+    if (tol < 0 || tol > 60) return fail(RPY_ERR_INVALID_ARGUMENT, "resolution out of range");
+    if (std::fabs(std::ldexp(inf, tol)) > 9.0e15 || std::fabs(std::ldexp(sup, tol)) > 9.0e15)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "interval too large for the resolution");
+    DI begin = from_param(inf, tol);
+    DI end = from_param(sup, tol);
+    std::vector<DI> left, right;
+    int guard = 0;
+    while (!contains(begin, end)) {
+        DI next = expand(begin);
+        if (!aligned(begin)) {
+            left.push_back(shrink_omitted(next));
+            next.k += 1;
+        }
+        begin = next;
+        if (++guard > 4096) return fail(RPY_ERR_INVALID_ARGUMENT, "dyadic dissection diverged");
+    }
+    DI next = end;
+    while (true) {
+        next = expand(next);
+        if (!contains(begin, next)) break;
+        if (!aligned(end)) right.push_back(shrink_contained(next));
+        end = next;
+        if (++guard > 8192) return fail(RPY_ERR_INVALID_ARGUMENT, "dyadic dissection diverged");
+    }
+    out = left;
+    out.insert(out.end(), right.rbegin(), right.rend());
+    return RPY_OK;
+}
+
+namespace {
+
+int64_t lower_bound_param(const double* p, int64_t n, double v)
+{
+    return int64_t(std::lower_bound(p, p + n, v) - p);
+}
+
+}  // namespace
+
+// Row ranges [b, e) of the sorted params that the query [inf, sup) at
+// `query_res` covers for a stream of resolution `stream_res`.
+int select_rows(const double* params, int64_t n, int32_t stream_res, double inf, double sup,
+                int32_t query_res, std::vector<std::pair<int64_t, int64_t>>& ranges)
+{
+    ranges.clear();
+    std::vector<DI> pieces;
+    int rc = dyadic_dissection(inf, sup, query_res, pieces);
+    if (rc) return rc;
+    for (const DI& d : pieces) {
+        const double lo = std::ldexp(double(d.k), -d.n);
+        const double hi = std::ldexp(double(d.k + 1), -d.n);
+        const int64_t b = lower_bound_param(params, n, lo);
+        const int64_t e = lower_bound_param(params, n, hi);
+        if (b == e) continue;  // empty piece contributes nothing
+        if (d.n <= stream_res) {
+            ranges.push_back({b, e});
+        } else {
+            // finer than the stream: the leaf counts iff the piece starts it
+            const DI leaf = rebase(d, stream_res);
+            const double llo = std::ldexp(double(leaf.k), -leaf.n);
+            if (llo == lo) {
+                const double lhi = std::ldexp(double(leaf.k + 1), -leaf.n);
+                ranges.push_back({lower_bound_param(params, n, llo),
+                                  lower_bound_param(params, n, lhi)});
+            }
+        }
+    }
+    return RPY_OK;
+}
+
+}  // namespace rpy_amd
+
+using namespace rpy_amd;
+
+extern "C" {
+
+int32_t rpy_param_to_resolution(double arg) { return param_to_resolution(arg); }
+
+int rpy_to_dyadic_intervals(double inf, double sup, int32_t tol, int64_t* k_out,
+                            int32_t* n_out, int32_t max_pieces, int32_t* count)
+{
+    if (!count) return fail(RPY_ERR_INVALID_ARGUMENT, "null count");
+    std::vector<DI> pieces;
+    int rc = dyadic_dissection(inf, sup, tol, pieces);
+    if (rc) return rc;
+    *count = int32_t(pieces.size());
+    if (int32_t(pieces.size()) > max_pieces)
+        return fail(RPY_ERR_SHAPE, "%zu pieces do not fit in %d", pieces.size(), max_pieces);
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        if (k_out) k_out[i] = pieces[i].k;
+        if (n_out) n_out[i] = pieces[i].n;
+    }
+    return RPY_OK;
+}
+
+int rpy_stream_select_rows(const double* params, int64_t n_params, int32_t stream_resolution,
+                           double inf, double sup, int32_t query_resolution, int64_t* ranges,
+                           int64_t max_ranges, int64_t* n_ranges)
+{
+    if (!n_ranges || (n_params > 0 && !params))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null argument");
+    for (int64_t i = 1; i < n_params; ++i)
+        if (!(params[i - 1] < params[i]))
+            return fail(RPY_ERR_INVALID_ARGUMENT, "params must be strictly increasing");
+    std::vector<std::pair<int64_t, int64_t>> r;
+    int rc = select_rows(params, n_params, stream_resolution, inf, sup, query_resolution, r);
+    if (rc) return rc;
+    *n_ranges = int64_t(r.size());
+    if (int64_t(r.size()) > max_ranges)
+        return fail(RPY_ERR_SHAPE, "%zu ranges do not fit in %lld", r.size(),
+                    (long long)max_ranges);
+    if (ranges)
+        for (size_t i = 0; i < r.size(); ++i) {
+            ranges[2 * i] = r[i].first;
+            ranges[2 * i + 1] = r[i].second;
+        }
+    return RPY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+"""Streams on the GPU: LieIncrementStream and batched signatures.
+
+* ``LieIncrementStream.from_increments(...)`` mirrors the reference
+  constructor (roughpy/src/streams/lie_increment_stream.cpp:73-245) and its
+  ``signature`` / ``log_signature`` queries mirror Stream.signature /
+  log_signature (roughpy/src/streams/stream.cpp:184-276, arguments as
+  signature_arguments.cpp:56-241): interval or (inf, sup), optional
+  resolution (int, or float converted by param_to_resolution), depth=.
+* ``signature_batch`` / ``log_signature_batch`` are the batched entry points
+  over many uneven streams (CSR offsets), one persistent GPU launch each.
+
+Which increments a query covers is decided on the host by the native
+library (rpy_stream_select_rows); the signature of the covered rows is one
+Chen-product launch on the GPU.
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _capi, _device
+from .bases import LieBasis, TensorBasis
+from .compute import DenseFreeTensor, DenseLie
+
+__all__ = [
+    "RealInterval", "LieIncrementStream", "signature_batch", "log_signature_batch",
+    "param_to_resolution", "to_dyadic_intervals",
+]
+
+
+def param_to_resolution(arg: float) -> int:
+    return int(_capi.lib().rpy_param_to_resolution(float(arg)))
+
+
+def to_dyadic_intervals(inf: float, sup: float, tol: int):
+    """Clopen [inf, sup) as maximal aligned dyadic pieces (k, n), left to right."""
+    L = _capi.lib()
+    import ctypes
+    count = ctypes.c_int32(0)
+    rc = L.rpy_to_dyadic_intervals(float(inf), float(sup), int(tol), None, None, 0,
+                                   ctypes.byref(count))
+    if rc not in (_capi.RPY_OK, _capi.RPY_ERR_SHAPE):
+        _capi.check(rc)
+    n = count.value
+    ks = np.zeros(max(n, 1), dtype=np.int64)
+    ns = np.zeros(max(n, 1), dtype=np.int32)
+    _capi.check(L.rpy_to_dyadic_intervals(float(inf), float(sup), int(tol), _capi.i64p(ks),
+                                           ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                           n, ctypes.byref(count)))
+    return [(int(ks[i]), int(ns[i])) for i in range(n)]
+
+
+class RealInterval:
+    """Clopen real interval [inf, sup)."""
+
+    __slots__ = ("inf", "sup")
+
+    def __init__(self, inf: float, sup: float):
+        self.inf, self.sup = float(inf), float(sup)
+
+    def __repr__(self):
+        return f"RealInterval({self.inf}, {self.sup})"
+
+
+def _resolution_arg(res) -> Optional[int]:
+    if res is None:
+        return None
+    if isinstance(res, bool):
+        raise TypeError("resolution should be either float or int")
+    if isinstance(res, (int, np.integer)):
+        return int(res)
+    if isinstance(res, (float, np.floating)):
+        return param_to_resolution(float(res))
+    raise TypeError("resolution should be either float or int")
+
+
+def _dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.float64:
+        return _capi.RPY_F64
+    if dt == np.float32:
+        return _capi.RPY_F32
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def signature_batch(increments, offsets, width: int, depth: int, dtype=None,
+                    compensated: bool = False):
+    """Signatures of uneven streams.
+
+    increments: (rows, width) array (NumPy or GPU tensor); offsets: (n+1,)
+    int64 CSR offsets, stream s owning rows offsets[s]:offsets[s+1].
+    Returns an (n, tensor_size) GPU tensor (NumPy if increments is NumPy).
+    """
+    dt = np.dtype(dtype) if dtype is not None else _device.np_dtype(increments)
+    if dt not in (np.float32, np.float64):
+        dt = np.dtype(np.float64)
+    code = _dtype_code(dt)
+    if compensated:
+        if dt != np.float32:
+            raise TypeError("compensated signatures take float32 data")
+        code = _capi.RPY_F32_COMPENSATED
+    inc = _device.to_device(increments, dt)
+    if inc.ndim != 2 or inc.shape[1] != width:
+        inc = inc.reshape(-1, width)
+    off = _device.to_device(np.asarray(offsets, dtype=np.int64) if isinstance(offsets, (list, tuple))
+                            else offsets)
+    t = _device.torch()
+    off = off.to(t.int64).contiguous()
+    n = int(off.shape[0]) - 1
+    if n < 0:
+        raise ValueError("offsets must hold at least one entry")
+    tsize = int(_capi.lib().rpy_tensor_size(int(width), int(depth)))
+    out = _device.empty((n, tsize), dt)
+    _capi.check(_capi.lib().rpy_stream_signature_batch(
+        out.data_ptr(), inc.data_ptr(), off.data_ptr(), n, int(width), int(depth), code,
+        _device.stream_handle()))
+    return _device.to_host(out) if isinstance(increments, np.ndarray) else out
+
+
+def log_signature_batch(increments, offsets, width: int, depth: int, dtype=None,
+                        lie_basis: Optional[LieBasis] = None):
+    """Log-signatures in the Hall basis (float64), one row per stream."""
+    dt = np.dtype(dtype) if dtype is not None else _device.np_dtype(increments)
+    if dt not in (np.float32, np.float64):
+        dt = np.dtype(np.float64)
+    code = _dtype_code(dt)
+    lie_basis = lie_basis or LieBasis(width, depth)
+    t2l = lie_basis.get_t2l_matrix(np.float64)
+    inc = _device.to_device(increments, dt).reshape(-1, width)
+    t = _device.torch()
+    off = _device.to_device(np.asarray(offsets, dtype=np.int64)).to(t.int64).contiguous()
+    n = int(off.shape[0]) - 1
+    out = _device.empty((n, lie_basis.size()), np.float64)
+    _capi.check(_capi.lib().rpy_stream_log_signature_batch(
+        out.data_ptr(), inc.data_ptr(), off.data_ptr(), n, int(width), int(depth), code,
+        t2l.device_struct(), _device.stream_handle()))
+    return _device.to_host(out) if isinstance(increments, np.ndarray) else out
+
+
+class LieIncrementStream:
+    """A stream given by its increments at parameter values (indices)."""
+
+    def __init__(self, params, rows, width, depth, resolution, support, dtype):
+        self._params = params          # strictly increasing float64
+        self._rows = rows              # (n, width) in dtype
+        self.width = int(width)
+        self.depth = int(depth)
+        self.resolution = int(resolution)
+        self.support = support         # (inf, sup)
+        self.dtype = np.dtype(dtype)
+        self._dev_rows = None
+
+    @classmethod
+    def from_increments(cls, data, indices=None, *, width=None, depth=None, resolution=None,
+                        dtype=None, coeffs=None, support=None, ctx=None, **kwargs):
+        """Mirror of LieIncrementStream.from_increments (lie_increment_stream.cpp:73-245).
+
+        Default indices are 0..N-1 with resolution 0; given indices without a
+        resolution use param_to_resolution(min index) + 1 (the reference's
+        running-minimum quirk).  Support is [min, max + 2^-resolution).  On a
+        repeated index the first row is kept.
+        """
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {sorted(kwargs)}")
+        if ctx is not None:
+            width = width if width is not None else ctx.width
+            depth = depth if depth is not None else ctx.depth
+        if isinstance(data, (list, tuple)) and data and hasattr(data[0], "data"):
+            data = np.stack([np.asarray(getattr(d, "data")) for d in data])
+        arr = np.asarray(data)
+        if dtype is None:
+            dtype = coeffs
+        if dtype is None:
+            dtype = np.float32 if arr.dtype == np.float32 else np.float64
+        dtype = np.dtype(dtype)
+        if arr.size == 0:
+            arr = np.zeros((0, int(width or 0)), dtype=dtype)
+        if arr.ndim == 1:
+            arr = arr.reshape(1, -1)
+        if width is None:
+            width = arr.shape[1]
+        width = int(width)
+        if arr.shape[1] < width:
+            raise ValueError(f"data has {arr.shape[1]} columns, width {width} requested")
+        arr = np.ascontiguousarray(arr[:, :width], dtype=dtype)
+        depth = int(depth if depth is not None else 2)
+        n = arr.shape[0]
+        if indices is None:
+            idx = np.arange(n, dtype=np.float64)
+            res = 0
+        else:
+            idx = np.asarray(indices, dtype=np.float64).reshape(-1)
+            if idx.shape[0] != n:
+                raise ValueError("mismatch between number of rows in data and number of indices")
+            res = resolution
+            if res is None and n:
+                min_diff = math.inf
+                for v in idx:  # `previous` stays 0 in the reference
+                    if v - 0.0 < min_diff:
+                        min_diff = v - 0.0
+                res = param_to_resolution(min_diff) + 1
+        # default indices force resolution 0 whatever was asked
+        # (roughpy/src/streams/lie_increment_stream.cpp:176-182)
+        res = int(res if res is not None else 0)
+        # first occurrence wins on a repeated index; rows sorted by index
+        uniq, first = np.unique(idx, return_index=True)
+        params = uniq.astype(np.float64)
+        rows = arr[first] if n else arr
+        if n:
+            sup_ = float(idx.max()) + math.ldexp(1.0, -res)
+            sup_support = (float(idx.min()), sup_)
+        else:
+            sup_support = (0.0, 0.0)
+        if support is not None:
+            sup_support = (float(support[0]), float(support[1])) if not isinstance(
+                support, RealInterval) else (support.inf, support.sup)
+        return cls(params, rows, width, depth, res, sup_support, dtype)
+
+    # -- queries -------------------------------------------------------------
+    def _parse(self, interval_or_inf, sup, resolution):
+        res = _resolution_arg(resolution)
+        if interval_or_inf is None:
+            return self.support, res
+        if isinstance(interval_or_inf, RealInterval):
+            if sup is not None:
+                res = _resolution_arg(sup)
+            return (interval_or_inf.inf, interval_or_inf.sup), res
+        if isinstance(interval_or_inf, (int, float, np.integer, np.floating)) and not isinstance(
+                interval_or_inf, bool):
+            if sup is None:
+                if isinstance(interval_or_inf, (float, np.floating)):
+                    raise TypeError("expecting integer resolution")
+                return self.support, int(interval_or_inf)
+            inf_, sup_ = float(interval_or_inf), float(sup)
+            if inf_ > sup_:
+                raise ValueError("inf must not be larger than sup")
+            return (inf_, sup_), res
+        if hasattr(interval_or_inf, "inf") and hasattr(interval_or_inf, "sup"):
+            inf_ = interval_or_inf.inf() if callable(interval_or_inf.inf) else interval_or_inf.inf
+            sup_ = interval_or_inf.sup() if callable(interval_or_inf.sup) else interval_or_inf.sup
+            if sup is not None:
+                res = _resolution_arg(sup)
+            return (float(inf_), float(sup_)), res
+        raise TypeError("unexpected type for argument interval_or_inf")
+
+    def _select(self, interval, resolution):
+        """Row indices the query covers (Stream::refine_interval + dyadic plan)."""
+        lo, hi = self.support
+        inf_, sup_ = interval
+        if sup_ < lo or inf_ > hi:
+            return np.zeros(0, dtype=np.int64)
+        a, b = max(inf_, lo), min(sup_, hi)
+        if b - a == 0.0:
+            return np.zeros(0, dtype=np.int64)
+        q_res = resolution if resolution is not None else max(
+            self.resolution, param_to_resolution(b - a) + 2)
+        L = _capi.lib()
+        n = self._params.shape[0]
+        params = np.ascontiguousarray(self._params, dtype=np.float64)
+        nr = np.zeros(1, dtype=np.int64)
+        cap = 4 * 64 + 8
+        ranges = np.zeros(2 * cap, dtype=np.int64)
+        _capi.check(L.rpy_stream_select_rows(_capi.dblp(params), n, self.resolution, a, b,
+                                             int(q_res), _capi.i64p(ranges), cap,
+                                             _capi.i64p(nr)))
+        sel = [np.arange(ranges[2 * i], ranges[2 * i + 1]) for i in range(int(nr[0]))]
+        return np.concatenate(sel).astype(np.int64) if sel else np.zeros(0, dtype=np.int64)
+
+    def signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
+                  dtype=None, ctx=None) -> DenseFreeTensor:
+        interval, res = self._parse(interval_or_inf, sup, resolution)
+        depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
+        rows = self._rows[self._select(interval, res)]
+        dt = np.dtype(dtype) if dtype is not None else self.dtype
+        basis = TensorBasis(self.width, depth)
+        if rows.shape[0] == 0:
+            data = np.zeros(basis.size(), dtype=dt)
+            data[0] = 1
+            return DenseFreeTensor(data, basis)
+        offs = np.array([0, rows.shape[0]], dtype=np.int64)
+        sig = signature_batch(np.ascontiguousarray(rows, dtype=dt), offs, self.width, depth,
+                              dtype=dt)
+        return DenseFreeTensor(sig[0], basis)
+
+    def log_signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
+                      dtype=None, ctx=None) -> DenseLie:
+        interval, res = self._parse(interval_or_inf, sup, resolution)
+        depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
+        rows = self._rows[self._select(interval, res)]
+        lb = LieBasis(self.width, depth)
+        if rows.shape[0] == 0:
+            return DenseLie(np.zeros(lb.size(), dtype=np.float64), lb)
+        offs = np.array([0, rows.shape[0]], dtype=np.int64)
+        ls = log_signature_batch(np.ascontiguousarray(rows), offs, self.width, depth,
+                                 dtype=self.dtype if dtype is None else dtype, lie_basis=lb)
+        return DenseLie(ls[0], lb)

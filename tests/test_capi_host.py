@@ -1,0 +1,237 @@
+"""The C-ABI library on the CPU: it loads without a GPU, exports every symbol
+include/roughpy_amd.h declares, and its host-side logic (Hall basis, l2t /
+t2l maps, dyadic dissection, stream row selection, and a host run of the
+signature kernel's lane decomposition) agrees with the oracle.  No compute
+entry point is called here."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+import roughpy_amd as ra
+from oracle import oracle as O
+from roughpy_amd import _capi
+from roughpy_amd.streams import LieIncrementStream, param_to_resolution, to_dyadic_intervals
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_goldens.json")))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "roughpy_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rpy_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _capi.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+        assert name in _capi.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_sizes():
+    L = _capi.lib()
+    assert L.rpy_version() >= 1
+    for w, d in [(3, 4), (4, 6), (5, 5), (8, 4), (6, 5)]:
+        assert L.rpy_tensor_size(w, d) == O.tensor_size(w, d)
+        assert L.rpy_lie_basis_size(w, d) == sum(O.witt_dim(w, k) for k in range(1, d + 1))
+    assert L.rpy_tensor_size(0, 3) == -1
+
+
+@pytest.mark.parametrize("w,d", [(2, 3), (3, 4), (5, 5), (8, 4), (6, 5), (4, 6), (2, 8)])
+def test_hall_basis_matches_oracle(w, d):
+    lb = ra.LieBasis(w, d)
+    db, data = O.hall_basis(w, d)
+    assert_array_equal(lb.degree_begin, db)
+    assert_array_equal(lb.data, data)
+
+
+def test_hall_golden_order():
+    lb = ra.LieBasis(2, 3)
+    assert [lb.key_to_str(k) for k in lb] == GOLD["hall_order_w2_d3"]["keys"]
+
+
+@pytest.mark.parametrize("w,d", [(2, 3), (3, 4), (4, 4), (2, 6)])
+def test_l2t_t2l_match_oracle(w, d):
+    lb = ra.LieBasis(w, d)
+    assert_array_equal(lb.get_l2t_matrix().to_dense(), O.l2t_dense(w, d))
+    assert_allclose(lb.get_t2l_matrix().to_dense(), O.t2l_dense(w, d), rtol=0, atol=1e-16)
+
+
+def test_l2t_golden_and_caching():
+    lb = ra.LieBasis(2, 3)
+    assert_array_equal(lb.get_l2t_matrix().to_dense(), GOLD["l2t_w2_d3"]["matrix"])
+    # tests/compute/test_l2t_and_t2l.py:59-75: caches are per map and per dtype
+    assert lb.get_l2t_matrix(np.float64) is lb.get_l2t_matrix(np.float64)
+    assert lb.get_t2l_matrix(np.float64) is lb.get_t2l_matrix(np.float64)
+    assert lb.get_t2l_matrix(np.float64) is not lb.get_l2t_matrix(np.float64)
+    assert lb.get_l2t_matrix(np.float32).data.dtype == np.float32
+
+
+def test_t2l_l2t_roundtrip_large():
+    lb = ra.LieBasis(5, 5)
+    l2t = lb.get_l2t_matrix().to_dense()
+    t2l = lb.get_t2l_matrix().to_dense()
+    assert_allclose(t2l @ l2t, np.eye(lb.size()), atol=1e-14)
+
+
+def test_check_lie_basis_data():
+    lb = ra.LieBasis(3, 4)
+    assert ra.check_lie_basis_data(lb.data, lb.degree_begin, 3, 4)
+    bad = np.array(lb.data)
+    bad[5] = (bad[5, 1], bad[5, 0])          # [k, l] with k > l
+    assert not ra.check_lie_basis_data(bad, lb.degree_begin, 3, 4)
+    with pytest.raises(ValueError):
+        ra.check_lie_basis_data(bad, lb.degree_begin, 3, 4, raise_on_fail=True)
+    with pytest.raises(ValueError):
+        ra.LieBasis(3, 4, degree_begin=lb.degree_begin, data=bad)
+    same = ra.LieBasis(3, 4, degree_begin=lb.degree_begin, data=lb.data)
+    assert same == lb and hash(same) == hash(lb)
+
+
+def test_tensor_basis_mirror():
+    tb = ra.TensorBasis(3, 4)
+    assert list(tb.degree_begin) == [0, 1, 4, 13, 40, 121]
+    assert tb.size() == 121
+    assert tb.truncate(2) == ra.TensorBasis(3, 2)
+    assert tb != ra.TensorBasis(4, 4)
+    assert len({ra.TensorBasis(2, 2), ra.TensorBasis(2, 2), ra.TensorBasis(2, 3)}) == 2
+    with pytest.raises(ValueError):
+        ra.TensorBasis(3, 4, degree_begin=[0, 1, 4, 13, 40, 120])
+
+
+@pytest.mark.parametrize("case", GOLD["to_dyadic_intervals"]["cases"])
+def test_dyadic_goldens(case):
+    assert to_dyadic_intervals(case["inf"], case["sup"], case["tol"]) == [
+        tuple(p) for p in case["pieces"]]
+
+
+def test_dyadic_matches_oracle_random(rng):
+    for _ in range(300):
+        a, b = sorted(rng.uniform(-3, 3, size=2))
+        tol = int(rng.integers(0, 14))
+        assert to_dyadic_intervals(a, b, tol) == O.to_dyadic_intervals(a, b, tol)
+
+
+def test_param_to_resolution():
+    for v in [0.1, 0.25, 0.5, 0.7, 1.0, 3.0, 1e-6, 0.0]:
+        assert param_to_resolution(v) == O.param_to_resolution(v)
+
+
+def _rows_oracle(s: O.IncrementStream, inf, sup, res):
+    """Rows the oracle's recursive dyadic-caching semantics include."""
+    picked = []
+    q = s.query_resolution(inf, sup)
+    if q is None:
+        return picked
+    a, b, r = q
+    if res is not None:
+        r = res
+    params = s.params
+
+    def piece_rows(k, n):
+        lo, hi = k / 2.0 ** n, (k + 1) / 2.0 ** n
+        inside = [i for i, p in enumerate(params) if lo <= p < hi]
+        if not inside:
+            return []
+        if n == s.resolution:
+            return inside
+        if n > s.resolution:
+            lk = k >> (n - s.resolution)
+            if lk / 2.0 ** s.resolution == lo:
+                return piece_rows(lk, s.resolution)
+            return []
+        return piece_rows(2 * k, n + 1) + piece_rows(2 * k + 1, n + 1)
+
+    for k, n in O.to_dyadic_intervals(a, b, r):
+        picked += piece_rows(k, n)
+    return picked
+
+
+def test_stream_row_selection_matches_oracle(rng):
+    for trial in range(40):
+        n = int(rng.integers(1, 30))
+        idx = np.sort(rng.uniform(0, 2, size=n))
+        res = int(rng.integers(0, 6)) if trial % 2 else None
+        data = rng.normal(size=(n, 2))
+        s_o = O.IncrementStream(data, 2, 2, indices=idx, resolution=res)
+        s_p = LieIncrementStream.from_increments(data, indices=idx, width=2, depth=2,
+                                                 resolution=res)
+        assert s_p.resolution == s_o.resolution
+        assert s_p.support == pytest.approx(s_o.support)
+        for _ in range(10):
+            a, b = sorted(rng.uniform(-0.5, 2.5, size=2))
+            qres = int(rng.integers(0, 9)) if rng.uniform() < 0.5 else None
+            got = list(s_p._select((a, b), qres))
+            assert got == _rows_oracle(s_o, a, b, qres)
+
+
+@pytest.mark.parametrize("w,d,p", [(2, 3, 1), (2, 3, 3), (3, 4, 1), (3, 4, 2), (5, 5, 2),
+                                   (5, 5, 3), (8, 4, 2), (8, 4, 3), (6, 5, 3), (4, 6, 3),
+                                   (4, 6, 4)])
+def test_kernel_lane_decomposition_on_host(rng, w, d, p):
+    """The signature kernel's per-lane algorithm and output index map, run on
+    the host, equals the oracle's Chen signature."""
+    L = _capi.lib()
+    inc = np.ascontiguousarray(rng.normal(size=(25, w)) / 5)
+    out = np.full(O.tensor_size(w, d), np.nan)
+    _capi.check(L.rpy_debug_signature_host_emulation(_capi.dblp(out), _capi.dblp(inc), 25, w,
+                                                     d, p))
+    assert not np.isnan(out).any(), "some coefficient was never written"
+    assert O.level_errors(out, O.signature(inc, w, d), w, d).max() < 1e-13
+
+
+def test_kernel_prefix_choice():
+    L = _capi.lib()
+    assert L.rpy_debug_signature_prefix(5, 5, 0) == 2
+    assert L.rpy_debug_signature_prefix(5, 5, 1) == 3
+    assert L.rpy_debug_signature_prefix(9, 3, 1) == 0      # generic kernel
+
+
+def test_compute_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from roughpy_amd import compute as rc
+    tb = ra.TensorBasis(2, 2)
+    a = rc.FreeTensor(np.zeros(7), tb)
+    with pytest.raises(RuntimeError, match="GPU"):
+        rc.ft_mul(a, a)
+    with pytest.raises(RuntimeError, match="GPU"):
+        LieIncrementStream.from_increments(np.ones((2, 2)), depth=2).signature()
+
+
+def test_error_codes_map_to_exceptions():
+    L = _capi.lib()
+    tb = _capi.TensorBasisC(0, 2, None)
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 1, 1, 1, 1, None)
+    assert rc == _capi.RPY_ERR_BASIS
+    with pytest.raises(ValueError, match="width"):
+        _capi.check(rc)
+    tb = _capi.TensorBasisC(2, 2, None)
+    rc = L.rpy_dense_ft_fma(None, None, None, 7, ctypes.byref(tb), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1), 1, 7, 7, 7,
+                            None)
+    assert rc == _capi.RPY_ERR_DTYPE
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(2, 1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 1, 7, 7, 7, None)
+    assert rc == _capi.RPY_ERR_INVALID_ARGUMENT
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 4, 6, 7, 7, None)
+    assert rc == _capi.RPY_ERR_SHAPE
+    # zero-size batch is a no-op that touches no pointer
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 0, 7, 7, 7, None)
+    assert rc == _capi.RPY_OK

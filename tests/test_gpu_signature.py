@@ -1,0 +1,183 @@
+"""GPU parity of batched signatures, log-signatures and stream queries.
+
+float64: within 1e-12 normwise per level of the oracle on the same inputs.
+float32: error against the float64 oracle on the same float32 inputs is
+stated and bounded (2e-5 here, lengths up to 1000).
+At the full benchmark size (16384 streams, lengths up to 4096) the checks are
+size-independent properties: level one equals the sum of increments, Chen's
+identity between two halves of a stream, agreement between the float32 and
+float64 kernels, and a sampled comparison with the oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+import roughpy_amd.compute as rc
+from oracle import oracle as O
+from roughpy_amd import streams as rs
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_goldens.json")))
+REF = np.load(os.path.join(HERE, "golden", "ref_vectors.npz"))
+
+
+def _uneven(rng, n, lo, hi, w, dtype):
+    lens = rng.integers(lo, hi + 1, size=n)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / np.sqrt(np.repeat(np.maximum(lens, 1), lens))[:, None])
+    return inc.astype(dtype), offs
+
+
+@pytest.mark.parametrize("w,d", [(2, 3), (3, 4), (5, 5), (8, 4), (6, 5), (4, 6), (2, 8), (9, 3)])
+def test_signature_batch_f64_vs_oracle(gpu, rng, w, d):
+    inc, offs = _uneven(rng, 37, 0, 60, w, np.float64)
+    got = rs.signature_batch(inc, offs, w, d)
+    ref = O.signature_batch(inc, offs, w, d, nthreads=8)
+    err = O.level_errors(got, ref, w, d).max()
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("w,d", [(3, 4), (5, 5), (8, 4), (6, 5)])
+def test_signature_batch_f32_error_vs_f64_oracle(gpu, rng, w, d):
+    inc, offs = _uneven(rng, 40, 1, 1000, w, np.float32)
+    got = rs.signature_batch(inc, offs, w, d)
+    ref = O.signature_batch(inc.astype(np.float64), offs, w, d, nthreads=8)
+    err = O.level_errors(got, ref, w, d).max()
+    print(f"f32 signature (w={w}, d={d}) max normwise level error vs f64 oracle: {err:.3e}")
+    assert err <= 2e-5, err
+
+
+def test_reference_signature_vectors(gpu):
+    for key in sorted({k.split("/")[0] for k in REF.files}):
+        w = int(key.split("_")[0][1:])
+        d = int(key.split("_")[1][1:])
+        inc = REF[key + "/sig_increments"]
+        offs = REF[key + "/sig_offsets"]
+        got = rs.signature_batch(inc, offs, w, d)
+        tol = 1e-12 if inc.dtype == np.float64 else 2e-6
+        assert O.level_errors(got, REF[key + "/signature"], w, d).max() <= tol
+
+
+def test_empty_and_single_streams(gpu, rng):
+    w, d = 5, 5
+    lens = np.array([0, 1, 0, 3, 0])
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = rng.normal(size=(int(offs[-1]), w))
+    got = rs.signature_batch(inc, offs, w, d)
+    unit = np.zeros(O.tensor_size(w, d))
+    unit[0] = 1
+    for s in (0, 2, 4):
+        assert_array_equal(got[s], unit)
+    assert O.level_errors(got[1], O.ft_exp(np.concatenate([[0], inc[0], np.zeros(unit.size - 1 - w)]), w, d), w, d).max() < 1e-14
+    assert rs.signature_batch(np.zeros((0, w)), np.zeros(1, np.int64), w, d).shape == (0, unit.size)
+
+
+def test_log_signature_batch_vs_oracle(gpu, rng):
+    for (w, d) in [(3, 4), (5, 5), (8, 4)]:
+        inc, offs = _uneven(rng, 9, 1, 200, w, np.float64)
+        got = rs.log_signature_batch(inc, offs, w, d)
+        for s in range(9):
+            ref = O.log_signature(inc[offs[s]:offs[s + 1]], w, d)
+            assert_allclose(got[s], ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
+def test_config1_single_stream(gpu):
+    """One stream, width 3, 1000 increments ~ N(0, 1/1000), depth 4: signature
+    (121) and log-signature (32) through LieIncrementStream."""
+    rng = np.random.default_rng(12345 + 1)
+    w, d, n = 3, 4, 1000
+    inc = rng.normal(0.0, np.sqrt(1.0 / n), size=(n, w))
+    s = rs.LieIncrementStream.from_increments(inc, width=w, depth=d)
+    sig = s.signature()
+    ref = O.signature(inc, w, d)
+    assert sig.data.shape == (121,)
+    assert O.level_errors(sig.data, ref, w, d).max() <= 1e-12
+    ls = s.log_signature()
+    lref = O.log_signature(inc, w, d)
+    assert ls.data.shape == (32,)
+    assert_allclose(ls.data, lref, rtol=0, atol=1e-12 * np.abs(lref).max())
+    # oracle stream semantics agree on sub-intervals
+    so = O.IncrementStream(inc, w, d)
+    for a, b in [(0, 1000), (10.5, 500.25), (999, 1000), (123, 124), (0.3, 0.7)]:
+        g = s.signature(a, b).data
+        assert O.level_errors(g, so.signature(a, b), w, d).max() <= 1e-12
+
+
+def test_stream_goldens_on_gpu(gpu):
+    g = GOLD["stream_tick_accuracy"]
+    s = rs.LieIncrementStream.from_increments(np.array(g["data"]), indices=np.array(g["indices"]),
+                                              width=2, depth=2)
+    unit = np.eye(1, 7).ravel()
+    q0, q1 = g["queries"]
+    assert_array_equal(s.signature(q0["inf"], q0["sup"], q0["resolution_param"]).data, unit)
+    arg = np.zeros(7)
+    arg[:3] = q1["exp_arg"]
+    assert_allclose(s.signature(q1["inf"], q1["sup"], q1["resolution_param"]).data,
+                    O.ft_exp(arg, 2, 2), rtol=1e-15)
+    g = GOLD["stream_interval_alignment"]
+    s = rs.LieIncrementStream.from_increments(np.array(g["data"]), depth=2)
+    assert_allclose(s.log_signature(rs.RealInterval(0.0, 0.1)).data[:3], g["first"]["expected_lie"])
+    for a, b in g["zero_windows"]:
+        assert_array_equal(s.log_signature(rs.RealInterval(a, b)).data, 0.0)
+    for c in GOLD["stream_from_array_level1"]["cases"]:
+        s = rs.LieIncrementStream.from_increments(np.array(c["data"]), width=c["width"],
+                                                  depth=c["depth"])
+        assert_allclose(s.signature(depth=c["sig_depth"]).data, c["expected"])
+    g = GOLD["linear_signature"]
+    x = np.array(g["increment"])
+    s = rs.LieIncrementStream.from_increments(x[None, :], indices=np.array([0.0]), width=3, depth=3)
+    l2 = np.einsum("i,j->ij", x, x) / 2
+    exp = np.hstack([[1.0], x, l2.ravel(), np.einsum("ij,k->ijk", l2, x).ravel() / 3])
+    assert_allclose(s.signature(rs.RealInterval(0.0, 1.0)).data, exp, rtol=1e-14)
+
+
+def test_randint_level_one(gpu, rng):
+    # tests/streams/test_lie_increment_path.py:290-332
+    arr = rng.integers(0, 5, size=(4, 3))
+    s = rs.LieIncrementStream.from_increments(arr, width=3, depth=2)
+    sig = s.signature(rs.RealInterval(0.0, 5.0), 2)
+    assert_array_equal(sig.data[:4], np.hstack([[1.0], arr.sum(axis=0)]))
+
+
+def test_benchmark_size_properties(gpu):
+    """16384 uneven streams, width 5, depth 5, lengths 64..4096 (the benchmark
+    workload) -- properties that need no oracle at this size, plus a sample."""
+    rng = np.random.default_rng(12345 + 3)
+    w, d, n = 5, 5, 16384
+    lens = rng.integers(64, 4097, size=n)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / np.sqrt(np.repeat(lens, lens))[:, None]).astype(np.float32)
+    t = gpu
+    dinc = t.from_numpy(inc).cuda()
+    doffs = t.from_numpy(offs).cuda()
+    s32 = rs.signature_batch(dinc, doffs, w, d)
+    s64 = rs.signature_batch(dinc.double(), doffs, w, d)
+    a32, a64 = s32.cpu().numpy(), s64.cpu().numpy()
+    # level one = sum of increments (float64 kernel)
+    sums = np.add.reduceat(inc.astype(np.float64), offs[:-1], axis=0)
+    assert_allclose(a64[:, 1:1 + w], sums, rtol=1e-12, atol=1e-12)
+    # float32 kernel agrees with the float64 kernel
+    err32 = O.level_errors(a32, a64, w, d).max()
+    print(f"benchmark workload f32 vs f64 kernel max normwise level error: {err32:.3e}")
+    assert err32 < 5e-5
+    # sampled exact comparison with the oracle
+    pick = rng.choice(n, size=6, replace=False)
+    for s in pick:
+        ref = O.signature(inc[offs[s]:offs[s + 1]].astype(np.float64), w, d)
+        assert O.level_errors(a64[s], ref, w, d).max() <= 1e-12
+    # Chen identity: split every stream in two and multiply the halves
+    mid = offs[:-1] + lens // 2
+    offs2 = np.empty(2 * n + 1, dtype=np.int64)
+    offs2[0:-1:2] = offs[:-1]
+    offs2[1::2] = mid
+    offs2[-1] = offs[-1]
+    halves = rs.signature_batch(dinc.double(), t.from_numpy(offs2).cuda(), w, d)
+    tb = rc.TensorBasis(w, d)
+    prod = rc.ft_mul(rc.FreeTensor(halves[0::2].contiguous(), tb),
+                     rc.FreeTensor(halves[1::2].contiguous(), tb))
+    assert O.level_errors(prod.data.cpu().numpy(), a64, w, d).max() <= 1e-12

@@ -1,0 +1,304 @@
+"""Pins the CPU oracle before anything is checked against it (CPU only).
+
+Each oracle function is checked against (a) the reference's own known-answer
+vectors (tests/golden/reference_goldens.json), (b) the reference's compiled
+kernels where the harness was built (tests/golden/ref_vectors.npz, written by
+tests/golden/make_ref_vectors.py from oracle/_ref/libref.so), and (c)
+closed forms / invariants / exact rational brute force, chosen so that a
+dropped term, a wrong sign or index, or a transposed operand fails.
+"""
+import json
+import os
+from fractions import Fraction
+from itertools import product
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+from oracle import oracle as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+REF = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
+
+
+# --- (a) reference known answers -------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ft_fma_explicit_golden(dtype):
+    g = GOLD["ft_fma_explicit"]
+    out = np.zeros(7, dtype=dtype)
+    O.ft_fma(out, np.array(g["b"], dtype), np.array(g["c"], dtype), 2, 2)
+    assert_array_equal(out, np.array(g["expected"], dtype))
+
+
+def test_ft_fma_construction_levels(rng):
+    # tests/compute/test_ft_fma.py:23-49 level by level
+    a, b, c = (rng.uniform(-1, 1, size=7) for _ in range(3))
+    exp = a.copy()
+    exp[0] += b[0] * c[0]
+    exp[1:3] += b[0] * c[1:3] + c[0] * b[1:3]
+    exp[3:] += b[0] * c[3:] + c[0] * b[3:] + np.outer(b[1:3], c[1:3]).flatten()
+    out = a.copy()
+    O.ft_fma(out, b, c, 2, 2)
+    assert_allclose(out, exp, rtol=0, atol=1e-15)
+
+
+def test_free_tensor_mul_golden():
+    g = GOLD["free_tensor_mul"]
+    r = O.ft_mul(np.array(g["t1"]), np.array(g["t2"]), 2, 2)
+    assert_array_equal(r, g["expected"])
+
+
+@pytest.mark.parametrize("name", ["exp_zero", "exp_letter"])
+def test_exp_goldens(name):
+    g = GOLD[name]
+    assert_array_equal(O.ft_exp(np.array(g["x"], float), 2, 2), g["expected"])
+
+
+def test_log_identity_golden():
+    g = GOLD["log_identity"]
+    assert_array_equal(O.ft_log(np.array(g["x"], float), 2, 2), g["expected"])
+
+
+@pytest.mark.parametrize("width", [2, 3, 4, 5])
+def test_exp_single_letter_depth3(width):
+    # tests/algebra/test_free_tensor.py:367-377
+    x = np.zeros(O.tensor_size(width, 3))
+    x[1] = 1.0
+    exp = np.zeros_like(x)
+    exp[0], exp[1], exp[1 + width], exp[1 + width + width ** 2] = 1, 1, 0.5, 1 / 6
+    assert_allclose(O.ft_exp(x, width, 3), exp, rtol=0, atol=1e-16)
+
+
+def test_hall_basis_golden_and_sizes():
+    g = GOLD["hall_order_w2_d3"]
+    db, data = O.hall_basis(2, 3)
+    assert [O.hall_key_str(data, k) for k in range(1, len(data))] == g["keys"]
+    for w, d, n in GOLD["lie_basis_sizes"]["cases"]:
+        db, data = O.hall_basis(w, d)
+        assert len(data) - 1 == n
+    # Witt dimensions at the benchmark shapes (level sizes from the formula)
+    for (w, d, sizes) in [(3, 4, [3, 3, 8, 18]), (5, 5, [5, 10, 40, 150, 624]),
+                          (8, 4, [8, 28, 168, 1008]), (6, 5, [6, 15, 70, 315, 1554]),
+                          (4, 6, [4, 6, 20, 60, 204, 670])]:
+        db, _ = O.hall_basis(w, d)
+        assert list(np.diff(db)[1:]) == sizes
+        assert [O.witt_dim(w, k) for k in range(1, d + 1)] == sizes
+
+
+def test_l2t_golden_matrix():
+    g = GOLD["l2t_w2_d3"]
+    assert_array_equal(O.l2t_dense(2, 3), np.array(g["matrix"], float))
+
+
+@pytest.mark.parametrize("case", GOLD["to_dyadic_intervals"]["cases"])
+def test_to_dyadic_intervals_goldens(case):
+    got = O.to_dyadic_intervals(case["inf"], case["sup"], case["tol"])
+    assert got == [tuple(p) for p in case["pieces"]]
+
+
+# --- (b) against the reference's compiled kernels ---------------------------
+
+_REF_KEYS = sorted({k.split("/")[0] for k in REF.files})
+
+
+@pytest.mark.parametrize("key", _REF_KEYS)
+def test_oracle_matches_reference_kernels(key):
+    w = int(key.split("_")[0][1:])
+    d = int(key.split("_")[1][1:])
+    g = {name.split("/")[1]: REF[name] for name in REF.files if name.startswith(key + "/")}
+    dt = g["a"].dtype
+    tol = 1e-14 if dt == np.float64 else 2e-6
+    fma = g["c"].copy()
+    O.ft_fma(fma, g["a"], g["b"], w, d)
+    assert_allclose(fma, g["fma"], rtol=tol, atol=tol)
+    im = g["a"].copy()
+    O.ft_inplace_mul(im, g["b"], w, d)
+    assert_allclose(im, g["inplace_mul"], rtol=tol, atol=tol)
+    assert_allclose(O.ft_exp(g["x"], w, d), g["exp"], rtol=tol, atol=tol)
+    assert_allclose(O.ft_fmexp(g["a"], g["x"], w, d), g["fmexp"], rtol=tol, atol=tol)
+    assert_allclose(O.ft_log(g["a"], w, d), g["log"], rtol=tol, atol=tol)
+    assert_array_equal(O.antipode(g["c"], w, d), g["antipode"])
+    assert_allclose(O.ft_adjoint_left_mul(g["a"], g["c"], w, d), g["adj_lmul"], rtol=tol, atol=tol)
+    sig = O.signature_batch(g["sig_increments"], g["sig_offsets"], w, d)
+    assert_allclose(sig, g["signature"], rtol=tol, atol=tol)
+
+
+# --- (c) mathematics ----------------------------------------------------------
+
+def _frac_mul(a, b, w, d):
+    db = O.tensor_degree_begin(w, d)
+    out = [Fraction(0)] * int(db[-1])
+    for la in range(d + 1):
+        for lb in range(d + 1 - la):
+            na, nb = w ** la, w ** lb
+            for i in range(na):
+                for j in range(nb):
+                    out[db[la + lb] + i * nb + j] += a[db[la] + i] * b[db[lb] + j]
+    return out
+
+
+def _frac_exp_increment(x, w, d):
+    # exp of a degree-one element: level k = x^(k) / k!
+    db = O.tensor_degree_begin(w, d)
+    out = [Fraction(0)] * int(db[-1])
+    out[0] = Fraction(1)
+    fact = 1
+    for k in range(1, d + 1):
+        fact *= k
+        for idx, word in enumerate(product(range(w), repeat=k)):
+            v = Fraction(1)
+            for a in word:
+                v *= x[a]
+            out[db[k] + idx] = v / fact
+    return out
+
+
+def test_signature_exact_rational_bruteforce():
+    w, d = 2, 3
+    incs = [[Fraction(1, 2), Fraction(-3, 4)], [Fraction(5, 8), Fraction(1, 8)],
+            [Fraction(-1, 4), Fraction(3, 2)]]
+    s = _frac_exp_increment([Fraction(0)] * w, w, d)
+    for x in incs:
+        s = _frac_mul(s, _frac_exp_increment(x, w, d), w, d)
+    got = O.signature(np.array([[float(v) for v in x] for x in incs]), w, d)
+    assert_allclose(got, [float(v) for v in s], rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("w,d", [(2, 4), (3, 3), (4, 3)])
+def test_chen_identity_and_log_exp(rng, w, d):
+    x = rng.normal(size=(7, w)) / 3
+    full = O.signature(x, w, d)
+    left, right = O.signature(x[:3], w, d), O.signature(x[3:], w, d)
+    assert_allclose(O.ft_mul(left, right, w, d), full, rtol=1e-13, atol=1e-14)
+    lg = O.ft_log(full, w, d)
+    assert_allclose(O.ft_exp(lg, w, d), full, rtol=1e-13, atol=1e-14)
+    # log of a group-like element is a Lie element: l2t(t2l(log S)) == log S
+    lie = O.tensor_to_lie(lg, w, d)
+    assert_allclose(O.lie_to_tensor(lie, w, d), lg, rtol=1e-12, atol=1e-14)
+
+
+def test_fmexp_and_antipode_invariants(rng):
+    w, d = 3, 3
+    a = O.ft_exp(np.concatenate([[0.0], rng.normal(size=w), np.zeros(9 + 27)]), w, d)
+    x = np.zeros_like(a)
+    x[1:1 + w] = rng.normal(size=w)
+    assert_allclose(O.ft_fmexp(a, x, w, d), O.ft_mul(a, O.ft_exp(x, w, d), w, d), atol=1e-14)
+    one = np.zeros_like(a)
+    one[0] = 1
+    assert_allclose(O.ft_fmexp(one, x, w, d), O.ft_exp(x, w, d), atol=1e-15)
+    # antipode of a group-like element is its inverse; antipode is an involution
+    assert_allclose(O.ft_mul(O.antipode(a, w, d), a, w, d), one, atol=1e-14)
+    y = rng.normal(size=a.shape)
+    assert_array_equal(O.antipode(O.antipode(y, w, d), w, d), y)
+
+
+def test_adjoint_left_mul_is_the_adjoint(rng):
+    # <A x, y> == <x, L_A^* y> for all x, y (brute force over a basis)
+    w, d = 2, 3
+    n = O.tensor_size(w, d)
+    A = rng.normal(size=n)
+    y = rng.normal(size=n)
+    lhs = np.array([np.dot(O.ft_mul(A, e, w, d), y) for e in np.eye(n)])
+    assert_allclose(O.ft_adjoint_left_mul(A, y, w, d), lhs, atol=1e-13)
+
+
+def test_t2l_inverts_l2t_and_matches_dynkin(rng):
+    w, d = 3, 4
+    l2t, t2l = O.l2t_dense(w, d), O.t2l_dense(w, d)
+    assert_allclose(t2l @ l2t, np.eye(l2t.shape[1]), atol=1e-15)
+    # l2t(t2l(word u)) = r(u) / |u| for every word (Dynkin)
+    db = O.tensor_degree_begin(w, d)
+    for k in (2, 3):
+        for idx, word in enumerate(product(range(w), repeat=k)):
+            col = l2t @ t2l[:, db[k] + idx]
+            assert_allclose(col[db[k]:db[k + 1]], O.dynkin_word(w, list(word)) / k, atol=1e-14)
+
+
+def test_constant_direction_closed_form():
+    # tests/streams/test_lie_increment_path.py:162-216
+    w, d = 3, 4
+    n = 5
+    t = np.arange(1.0, w + 1) * (2.0 / n)
+    sig = O.signature(np.tile(t, (n, 1)), w, d)
+    letters = np.arange(1, w + 1, dtype=float)
+    exp = [1.0]
+    fact = 1.0
+    for k in range(1, d + 1):
+        fact *= k
+        for word in product(range(w), repeat=k):
+            exp.append(np.prod(letters[list(word)]) * 2.0 ** k / fact)
+    assert_allclose(sig, exp, rtol=1e-13)
+
+
+def test_linear_signature_golden():
+    g = GOLD["linear_signature"]
+    x = np.array(g["increment"])
+    sig = O.signature(x[None, :], 3, 3)
+    l2 = np.einsum("i,j->ij", x, x) / 2
+    l3 = np.einsum("ij,k->ijk", l2, x) / 3
+    assert_allclose(sig, np.hstack([[1.0], x, l2.ravel(), l3.ravel()]), rtol=1e-14)
+
+
+def test_treelike_paths():
+    for pair in GOLD["treelike_paths"]["pairs"]:
+        a = np.diff(np.array(pair["tree"], float), axis=0)
+        b = np.diff(np.array(pair["pruned"], float), axis=0)
+        sa = O.signature(a, 2, 2)
+        sb = O.signature(b, 2, 2) if b.size else np.eye(1, 7).ravel()
+        assert_allclose(sa, sb, atol=1e-12)
+
+
+# --- stream query semantics -------------------------------------------------
+
+def test_stream_tick_accuracy_golden():
+    g = GOLD["stream_tick_accuracy"]
+    s = O.IncrementStream(np.array(g["data"]), 2, 2, indices=np.array(g["indices"]))
+    for q in g["queries"]:
+        r = s.signature(q["inf"], q["sup"], O.param_to_resolution(q["resolution_param"]))
+        if q["expected"] == "unit":
+            assert_array_equal(r, np.eye(1, 7).ravel())
+        else:
+            arg = np.zeros(7)
+            arg[:3] = q["exp_arg"]
+            assert_allclose(r, O.ft_exp(arg, 2, 2), rtol=1e-15)
+
+
+def test_stream_interval_alignment_golden():
+    g = GOLD["stream_interval_alignment"]
+    s = O.IncrementStream(np.array(g["data"]), 3, 2)
+    assert_allclose(s.log_signature(g["first"]["inf"], g["first"]["sup"])[:3],
+                    g["first"]["expected_lie"])
+    for a, b in g["zero_windows"]:
+        assert_allclose(s.log_signature(a, b), 0.0, atol=0)
+        assert_allclose(s.log_signature(a, b, g["set_resolution"]), 0.0, atol=0)
+
+
+def test_stream_sequence_and_level_one():
+    g = GOLD["stream_sequence_of_lies"]
+    s = O.IncrementStream(np.array(g["data"]), 3, 3)
+    assert_allclose(s.log_signature(g["inf"], g["sup"])[:3], g["expected_lie"])
+    for c in GOLD["stream_from_array_level1"]["cases"]:
+        data = np.array(c["data"], float)[:, : c["width"]]
+        s = O.IncrementStream(data, c["width"], c["sig_depth"])
+        lo, hi = s.support
+        assert_allclose(s.signature(lo, hi), c["expected"])
+
+
+# --- planted mistakes: each pin above must catch them ------------------------
+
+def test_planted_mistakes_are_caught():
+    g = GOLD["ft_fma_explicit"]
+    b, c = np.array(g["b"], float), np.array(g["c"], float)
+    good = O.ft_mul(b, c, 2, 2)
+    transposed = O.ft_mul(c, b, 2, 2)          # operands swapped
+    assert np.array_equal(good, g["expected"])
+    assert not np.array_equal(transposed, g["expected"])
+    x = np.zeros(O.tensor_size(3, 3))
+    x[1] = 1.0
+    e = O.ft_exp(x, 3, 3)
+    dropped_factorial = e.copy()
+    dropped_factorial[1 + 3] *= 2              # x^2 without the 1/2!
+    assert not np.allclose(dropped_factorial, [1, 1, 0, 0, 0.5] + [0] * 8 + [1 / 6] + [0] * 26)
+    assert np.allclose(e, [1, 1, 0, 0, 0.5] + [0] * 8 + [1 / 6] + [0] * 26)
