@@ -29,6 +29,21 @@ FMA_PER_STEP = sum((D - m + 1) * W ** m for m in range(1, D + 1))
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X published vector peaks
 
 
+PROFILE = os.path.join(ROOT, "profiles", "chen_signature_w5_d5_16384streams.md")
+
+
+def profiled_traffic(dtype):
+    """HBM bytes per launch of the signature kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith(f"traffic_bytes_{dtype} = "):
+                return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def make_workload(seed=SEED):
     rng = np.random.default_rng(seed)
     lens = rng.integers(LMIN, LMAX + 1, size=N_STREAMS)
@@ -52,6 +67,47 @@ def time_kernel(sig_fn, steps, warmup, torch):
     stop.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / steps
+
+
+def dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs, n_queries=100000, levels=6,
+                     resolution=12):
+    """Dyadic cache of the same batch at resolution 12 (stored levels 0..6)
+    and 100000 random clopen queries snapped to the 2^-12 grid."""
+    from roughpy_amd import streams as rs
+    rng = np.random.default_rng(SEED + 2)
+    params = np.concatenate([np.arange(L, dtype=np.float64) / L for L in lens])
+    dparams = torch.from_numpy(params).cuda()
+    sid = torch.from_numpy(rng.integers(0, N_STREAMS, size=n_queries)).cuda()
+    a = np.floor(rng.uniform(0, 1, size=n_queries) * 2 ** resolution) / 2 ** resolution
+    b = np.floor(rng.uniform(0, 1, size=n_queries) * 2 ** resolution) / 2 ** resolution
+    qa = torch.from_numpy(np.minimum(a, b)).cuda()
+    qb = torch.from_numpy(np.maximum(a, b)).cuda()
+    t0 = time.perf_counter()
+    cache = rs.DyadicSignatureCache(dinc32, dparams, doffs, W, D, resolution=resolution,
+                                    levels=levels)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    out = cache.query(sid, qa, qb)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(3):
+        cache.query(sid, qa, qb, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    q_ms = ev0.elapsed_time(ev1) / 3
+    t0 = time.perf_counter()
+    cache.build()
+    torch.cuda.synchronize()
+    rebuild_ms = (time.perf_counter() - t0) * 1e3
+    res = {"levels_stored": levels, "resolution": resolution, "dtype": "f32",
+           "cache_bytes": int(cache.cache.numel()) * 4, "build_ms": rebuild_ms,
+           "first_build_ms": build_ms, "queries": n_queries, "query_ms": q_ms,
+           "queries_per_s": n_queries / (q_ms * 1e-3)}
+    del cache, out
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(inc, offs, target_seconds=10.0):
@@ -94,6 +150,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device-to-device copy, the reference "
+                         "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -106,6 +165,12 @@ def main():
     from roughpy_amd import _device
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     inc, offs, lens = make_workload()
     total_incr = int(offs[-1])
     dinc32 = torch.from_numpy(inc).cuda()
@@ -174,7 +239,7 @@ def main():
             "peak": PEAK_TFLOPS[args.dtype],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS[args.dtype],
-            "traffic": None,
+            "traffic": profiled_traffic(args.dtype),
         },
         "f64_kernel": {"ms_per_step": results["f64"][0],
                        "value": N_STREAMS / (results["f64"][0] * 1e-3),
@@ -184,6 +249,7 @@ def main():
                        "tflops": flops / (results["f32"][0] * 1e-3) / 1e12},
         "rel_err_f32_vs_f64": err,
     }
+    line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(inc, offs)
     print(json.dumps(line), flush=True)

@@ -223,6 +223,44 @@ int rpy_stream_log_signature_batch(double *out, const void *increments,
                                    const rpy_sparse_matrix *t2l,
                                    void *stream);
 
+/* ---- dyadic signature cache (device) ------------------------------------ */
+
+/* Elements (of the cache dtype) of a dyadic cache storing node signatures of
+ * powers 0..levels over [0, 1) for n_streams streams; -1 on bad arguments. */
+int64_t rpy_dyadic_cache_elements(int64_t n_streams, int32_t width,
+                                  int32_t depth, int32_t levels);
+
+/* Builds the cache: node (j, s, k) = signature of the rows of stream s whose
+ * param lies in [k 2^-j, (k+1) 2^-j), for j = 0..levels, k < 2^j.  Rows of
+ * stream s are [offsets[s], offsets[s+1]) with strictly increasing params
+ * (device, float64; rows outside [0, 1) are never covered).  The reference
+ * memoises resolution leaves and rebuilds coarser nodes with CBH on every
+ * query (streams/src/dyadic_caching_layer.cpp:86-130); this cache builds
+ * every stored level once, in one fused pass: one signature launch for level
+ * `levels`, then one batched product per coarser level.  cache: device,
+ * rpy_dyadic_cache_elements(...) elements of dtype (RPY_F32 / RPY_F64). */
+int rpy_dyadic_cache_build(void *cache, const void *increments,
+                           const double *params, const int64_t *offsets,
+                           int64_t n_streams, int32_t width, int32_t depth,
+                           int32_t resolution, int32_t levels, rpy_dtype dtype,
+                           void *stream);
+
+/* Answers n_queries clopen interval queries [query_inf[q], query_sup[q]) on
+ * stream query_stream[q] at `resolution` (all device arrays): out[q] is the
+ * signature Stream.signature(RealInterval(inf, sup), resolution) returns for
+ * a LieIncrementStream of that resolution (streams/src/stream.cpp:107-171):
+ * the ordered Chen product over the maximal dyadic pieces of
+ * [floor(inf 2^r), floor(sup 2^r)) 2^-r, read from the cache for pieces of
+ * power <= levels and computed from the rows for finer ones.  An empty or
+ * out-of-range query gives the unit tensor. */
+int rpy_dyadic_cache_query(void *out, const void *cache, const void *increments,
+                           const double *params, const int64_t *offsets,
+                           int64_t n_streams, int32_t width, int32_t depth,
+                           int32_t resolution, int32_t levels,
+                           const int64_t *query_stream, const double *query_inf,
+                           const double *query_sup, int64_t n_queries,
+                           rpy_dtype dtype, void *stream);
+
 /* ---- stream queries (host) ---------------------------------------------- */
 
 /* -min(0, e-1) where arg = m 2^e, 0.5 <= |m| < 1; mirror of
@@ -263,6 +301,12 @@ int rpy_debug_signature_host_emulation(double *out, const double *increments,
  * the shape runs on the generic kernel. */
 int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth,
                                    int32_t is_f64);
+
+/* Host run of the query kernel's dissection of [begin, end) (units of
+ * 2^-resolution) into maximal aligned pieces; returns the piece count. */
+int32_t rpy_debug_dyadic_cover(int64_t begin, int64_t end, int32_t resolution,
+                               int64_t *k_out, int32_t *n_out,
+                               int32_t max_pieces);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,12 @@ SIGNATURES = {
     "rpy_debug_signature_host_emulation": (ctypes.c_int, [_p_dbl, _p_dbl, _i64, _i32, _i32,
                                                           _i32]),
     "rpy_debug_signature_prefix": (_i32, [_i32, _i32, _i32]),
+    "rpy_dyadic_cache_elements": (_i64, [_i64, _i32, _i32, _i32]),
+    "rpy_dyadic_cache_build": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32,
+                                              _i32, ctypes.c_int, _vp]),
+    "rpy_dyadic_cache_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32,
+                                              _i32, _vp, _vp, _vp, _i64, ctypes.c_int, _vp]),
+    "rpy_debug_dyadic_cover": (_i32, [_i64, _i64, _i32, _p_i64, ctypes.POINTER(_i32), _i32]),
 }
 
 

@@ -278,3 +278,39 @@ int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth, int32_t is_f64)
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int64_t rpy_dyadic_cache_elements(int64_t n_streams, int32_t width, int32_t depth, int32_t levels)
+{
+    return dyadic_cache_elements(n_streams, width, depth, levels);
+}
+
+int rpy_dyadic_cache_build(void* cache, const void* increments, const double* params,
+                           const int64_t* offsets, int64_t n_streams, int32_t width,
+                           int32_t depth, int32_t resolution, int32_t levels, rpy_dtype dtype,
+                           void* stream)
+{
+    return dyadic_cache_build(cache, increments, params, offsets, n_streams, width, depth,
+                              resolution, levels, dtype, stream);
+}
+
+int rpy_dyadic_cache_query(void* out, const void* cache, const void* increments,
+                           const double* params, const int64_t* offsets, int64_t n_streams,
+                           int32_t width, int32_t depth, int32_t resolution, int32_t levels,
+                           const int64_t* query_stream, const double* query_inf,
+                           const double* query_sup, int64_t n_queries, rpy_dtype dtype,
+                           void* stream)
+{
+    return dyadic_cache_query(out, cache, increments, params, offsets, n_streams, width, depth,
+                              resolution, levels, query_stream, query_inf, query_sup, n_queries,
+                              dtype, stream);
+}
+
+int32_t rpy_debug_dyadic_cover(int64_t begin, int64_t end, int32_t resolution, int64_t* k_out,
+                               int32_t* n_out, int32_t max_pieces)
+{
+    return dyadic_cover_host(begin, end, resolution, k_out, n_out, max_pieces);
+}
+
+}  // extern "C"

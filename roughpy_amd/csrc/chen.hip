@@ -28,40 +28,94 @@ namespace {
 constexpr int kWave = 64;
 
 // keys[i] = length of stream i, vals[i] = i; also clears the queue counter.
-__global__ void prep_lengths_kernel(const int64_t* __restrict__ offsets, int64_t n,
+// Stream i owns rows [begins[i], ends[i]) (CSR offsets: ends = offsets + 1).
+__global__ void prep_lengths_kernel(const int64_t* __restrict__ begins,
+                                    const int64_t* __restrict__ ends, int64_t n,
                                     uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
                                     uint32_t* __restrict__ counter)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i == 0) *counter = 0u;
     if (i < n) {
-        const int64_t len = offsets[i + 1] - offsets[i];
+        const int64_t len = ends[i] - begins[i];
         keys[i] = uint32_t(len < 0 ? 0 : (len > 0xffffffffLL ? 0xffffffffLL : len));
         vals[i] = int32_t(i);
     }
 }
 
+// Increment staging.  A chunk of 64 consecutive (stream, prefix) items spans
+// at most kSlots streams; their rows are copied block by block (kBlk steps)
+// into LDS with coalesced loads, the next block being fetched into registers
+// while the current one is consumed, so the step loop reads only LDS.
+template <typename T, int W, int NP>
+struct Staging {
+    static constexpr int kSlots = (kWave + NP - 2) / NP + 1;
+    // at most 3 prefetched values per lane: the lane state owns the VGPRs
+    static constexpr int kMaxPerLane = 3;
+    static constexpr int kBlkFit = kMaxPerLane * kWave / (kSlots * W);
+    static constexpr int kBlk = kBlkFit < 32 ? (kBlkFit < 1 ? 1 : kBlkFit) : 32;
+    static constexpr int kVals = kSlots * kBlk * W;        // raw values per block
+    static constexpr int kPerLane = (kVals + kWave - 1) / kWave;
+};
+
 template <typename T, int W, int D, int P>
-__global__ __launch_bounds__(kWave) void chen_persistent_kernel(
-        T* __restrict__ out, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
-        const int32_t* __restrict__ order, int64_t n_items, int64_t tsize, DevBasis b,
-        uint32_t* __restrict__ counter)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
+void chen_persistent_kernel(
+        T* __restrict__ out, const T* __restrict__ incr, const int64_t* __restrict__ begins,
+        const int64_t* __restrict__ ends, const int32_t* __restrict__ order, int64_t n_items,
+        int64_t tsize, DevBasis b, uint32_t* __restrict__ counter)
 {
     using Sh = chen::Shape<W, D, P>;
     constexpr int NP = Sh::NP;
+    using St = Staging<T, W, NP>;
+    constexpr int kSlots = St::kSlots;
+    constexpr int kBlk = St::kBlk;
+    constexpr int kBlkVals = kBlk * W;       // raw values of one slot's block
+    // Scaled rows (x_a / d, d = 1..D) unless the lane is wide: a float64 lane
+    // with P >= 3 would read more LDS bytes per step than the CU can serve.
+    constexpr bool kScaled = !(sizeof(T) == 8 && P >= 3);
+    constexpr int kRow = kScaled ? Sh::ROW : W;
+
+    // stage[(slot * kBlk + r) * kRow + (d-1) * W + a] = x_a(row r) / d
+    __shared__ T stage[kSlots * kBlk * kRow];
+    __shared__ int64_t slot_row0[kSlots];
+    __shared__ int64_t slot_len[kSlots];
+    __shared__ int32_t slot_stream[kSlots];
+
     const int lane = threadIdx.x;
+    const int64_t n_streams = n_items / NP;
     while (true) {
         uint32_t chunk = 0;
         if (lane == 0) chunk = atomicAdd(counter, 1u);
         chunk = __shfl(chunk, 0);
-        const int64_t item = int64_t(chunk) * kWave + lane;
-        if (int64_t(chunk) * kWave >= n_items) break;
+        const int64_t first_item = int64_t(chunk) * kWave;
+        if (first_item >= n_items) break;
+        const int64_t last_item = min(first_item + kWave, n_items) - 1;
+        const int64_t si_first = first_item / NP;
+        const int n_slots = int(last_item / NP - si_first) + 1;  // <= kSlots
+
+        __syncthreads();  // previous chunk is done with the slot table
+        if (lane < kSlots) {
+            int64_t r0 = 0, ln = 0;
+            int32_t s = 0;
+            if (lane < n_slots) {
+                s = order[si_first + lane];
+                r0 = begins[s];
+                ln = ends[s] - r0;
+            }
+            slot_row0[lane] = r0;
+            slot_len[lane] = ln;
+            slot_stream[lane] = s;
+        }
+        __syncthreads();
+
+        const int64_t item = first_item + lane;
         const bool active = item < n_items;
-        const int64_t si = active ? item / NP : 0;
+        const int slot = active ? int(item / NP - si_first) : 0;
         const int q = active ? int(item % NP) : 0;
-        const int s = active ? order[si] : 0;
-        const int64_t row0 = active ? offsets[s] : 0;
-        const int64_t len = active ? offsets[s + 1] - row0 : 0;
+        const int len = active ? int(slot_len[slot]) : 0;
+        // streams are sorted by length, longest first: slot 0 is the longest
+        const int maxlen = int(slot_len[0]);
 
         int u[P];
         {
@@ -75,17 +129,53 @@ __global__ __launch_bounds__(kWave) void chen_persistent_kernel(
         chen::Lane<T, W, D, P> st;
         st.init_identity();
 
-        const T* rp = incr + row0 * W;
-        for (int64_t t = 0; t < len; ++t, rp += W) {
-            T x[W];
-            T xu[P];
+        // staging assignment of this lane: values v = lane + 64 i
+        T pre[St::kPerLane];
+        auto fetch = [&](int t0) {
 #pragma unroll
-            for (int a = 0; a < W; ++a) x[a] = rp[a];
+            for (int i = 0; i < St::kPerLane; ++i) {
+                const int v = lane + kWave * i;
+                const int sl = v / kBlkVals;
+                const int rem = v - sl * kBlkVals;
+                const int r = rem / W;
+                T val = T(0);
+                if (v < St::kVals && sl < n_slots && t0 + r < slot_len[sl])
+                    val = incr[(slot_row0[sl] + t0) * W + rem];
+                pre[i] = val;
+            }
+        };
+
+        fetch(0);
+        for (int t0 = 0; t0 < maxlen; t0 += kBlk) {
 #pragma unroll
-            for (int j = 0; j < P; ++j) xu[j] = rp[u[j]];
-            st.step(x, xu);
+            for (int i = 0; i < St::kPerLane; ++i) {
+                const int v = lane + kWave * i;
+                if (v < St::kVals) {
+                    const int sl = v / kBlkVals;
+                    const int rem = v - sl * kBlkVals;
+                    const int r = rem / W;
+                    const int a = rem - r * W;
+                    T* dst = stage + (sl * kBlk + r) * kRow + a;
+                    chen::static_for<kScaled ? D : 1>([&](auto d0) {
+                        constexpr int d = int(decltype(d0)::value) + 1;
+                        dst[(d - 1) * W] = (d == 1) ? pre[i] : pre[i] * (T(1) / T(d));
+                    });
+                }
+            }
+            __syncthreads();
+            if (t0 + kBlk < maxlen) fetch(t0 + kBlk);
+            const int nsteps = min(kBlk, len - t0);
+            const T* rows = stage + slot * kBlk * kRow;
+            for (int tt = 0; tt < nsteps; ++tt, rows += kRow) {
+                if constexpr (kScaled)
+                    st.step(chen::ScaledRow<T, W, D, P>{rows, u});
+                else
+                    st.step(chen::RawRow<T, W, D, P>(rows, u));
+            }
+            __syncthreads();
         }
 
+        const int s = slot_stream[slot];
         if (active) {
             T* o = out + int64_t(s) * tsize;
             // subtree levels: lane q owns words q*W^(k-P) .. (q+1)*W^(k-P)-1
@@ -111,7 +201,8 @@ __global__ __launch_bounds__(kWave) void chen_persistent_kernel(
 template <typename T>
 __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
                                                           const T* __restrict__ incr,
-                                                          const int64_t* __restrict__ offsets,
+                                                          const int64_t* __restrict__ begins,
+                                                          const int64_t* __restrict__ ends,
                                                           int64_t n_streams, int64_t tsize,
                                                           DevBasis b)
 {
@@ -120,7 +211,7 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
         T* S = out + s * tsize;
         for (int64_t i = threadIdx.x; i < tsize; i += blockDim.x) S[i] = (i == 0) ? T(1) : T(0);
         __syncthreads();
-        for (int64_t r = offsets[s]; r < offsets[s + 1]; ++r) {
+        for (int64_t r = begins[s]; r < ends[s]; ++r) {
             const T* x = incr + r * W;
             for (int m = D; m >= 1; --m) {
                 const int64_t n = b.level_size(m);
@@ -132,7 +223,7 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
                         const int64_t pre = idx / scale;  // prefix of length K
                         const int letter = int(rest / scale);
                         rest -= int64_t(letter) * scale;
-                        p = __builtin_fma(p, x[letter] / T(m - K + 1), S[b.db[K] + pre]);
+                        p = p * (x[letter] / T(m - K + 1)) + S[b.db[K] + pre];
                         scale = (scale >= W) ? scale / W : 1;
                     }
                     // only this thread reads S_m[idx]; lower levels stay old
@@ -145,12 +236,13 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
     }
 }
 
-using Launcher = int (*)(void* out, const void* incr, const int64_t* offsets,
+using Launcher = int (*)(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
                          const int32_t* order, int64_t n_streams, const DevBasis& b,
                          uint32_t* counter, hipStream_t st);
 
 template <typename T, int W, int D, int P>
-int launch_persistent(void* out, const void* incr, const int64_t* offsets, const int32_t* order,
+int launch_persistent(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
+                      const int32_t* order,
                       int64_t n_streams, const DevBasis& b, uint32_t* counter, hipStream_t st)
 {
     constexpr int NP = chen::Shape<W, D, P>::NP;
@@ -164,7 +256,7 @@ int launch_persistent(void* out, const void* incr, const int64_t* offsets, const
     const int64_t chunks = (n_items + kWave - 1) / kWave;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(chunks, int64_t(cus) * per_cu));
     hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P>), dim3(uint32_t(grid)), dim3(kWave), 0,
-                       st, (T*)out, (const T*)incr, offsets, order, n_items, b.size(), b,
+                       st, (T*)out, (const T*)incr, begins, ends, order, n_items, b.size(), b,
                        counter);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
@@ -234,9 +326,9 @@ int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64)
     return dtype_is_f64 ? e->p64 : e->p32;
 }
 
-int stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
-                           int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
-                           void* stream)
+int stream_signature_ranges(void* out, const void* increments, const int64_t* begins,
+                            const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
+                            rpy_dtype dtype, void* stream)
 {
     if (dtype == RPY_F32_COMPENSATED)
         return fail(RPY_ERR_UNSUPPORTED, "compensated float32 signatures are not available yet");
@@ -250,7 +342,7 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
     if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
     if (n_streams == 0) return RPY_OK;
     if (n_streams > INT32_MAX) return fail(RPY_ERR_SHAPE, "too many streams");
-    if (!out || !offsets || !increments)
+    if (!out || !begins || !ends || !increments)
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 
@@ -259,11 +351,11 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
         const int grid = int(std::min<int64_t>(n_streams, 4096));
         if (dtype == RPY_F64)
             hipLaunchKernelGGL(chen_generic_kernel<double>, dim3(grid), dim3(256), 0, st,
-                               (double*)out, (const double*)increments, offsets, n_streams,
+                               (double*)out, (const double*)increments, begins, ends, n_streams,
                                b.size(), b);
         else
             hipLaunchKernelGGL(chen_generic_kernel<float>, dim3(grid), dim3(256), 0, st,
-                               (float*)out, (const float*)increments, offsets, n_streams,
+                               (float*)out, (const float*)increments, begins, ends, n_streams,
                                b.size(), b);
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
@@ -290,7 +382,7 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
 
     const int pb = 256;
     hipLaunchKernelGGL(prep_lengths_kernel, dim3(uint32_t((n + pb - 1) / pb)), dim3(pb), 0, st,
-                       offsets, n_streams, keys_in, vals_in, counter);
+                       begins, ends, n_streams, keys_in, vals_in, counter);
     rc = RPY_OK;
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "length prep launch failed: %s", hipGetErrorString(err));
@@ -301,11 +393,21 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
     }
     if (!rc) {
         Launcher f = (dtype == RPY_F64) ? e->f64 : e->f32;
-        rc = f(out, increments, offsets, vals_out, n_streams, b, counter, st);
+        rc = f(out, increments, begins, ends, vals_out, n_streams, b, counter, st);
     }
     hipError_t ferr = hipFreeAsync(ws, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ferr));
     return rc;
+}
+
+int stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
+                           int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
+                           void* stream)
+{
+    if (!offsets && n_streams > 0) return fail(RPY_ERR_INVALID_ARGUMENT, "null offsets");
+    // CSR offsets: stream s owns rows [offsets[s], offsets[s+1])
+    return stream_signature_ranges(out, increments, offsets, offsets ? offsets + 1 : nullptr,
+                                   n_streams, width, depth, dtype, stream);
 }
 
 // Runs the lane algorithm of the specialised kernel on the host for one
@@ -324,11 +426,13 @@ static void emulate(double* out, const double* incr, int64_t len, const DevBasis
         }
         chen::Lane<double, W, D, P> st;
         st.init_identity();
+        double row[Sh::ROW];
         for (int64_t r = 0; r < len; ++r) {
-            double x[W], xu[P];
-            for (int a = 0; a < W; ++a) x[a] = incr[r * W + a];
-            for (int j = 0; j < P; ++j) xu[j] = incr[r * W + u[j]];
-            st.step(x, xu);
+            chen::build_row<double, W, D>(row, incr + r * W);
+            if (r % 2)
+                st.step(chen::ScaledRow<double, W, D, P>{row, u});
+            else
+                st.step(chen::RawRow<double, W, D, P>(row, u));
         }
         for (int k = P; k <= D; ++k) {
             const int nk = int(ipow(W, k - P));

@@ -11,6 +11,11 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
                            int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
                            void* stream);
 
+// Same with explicit row ranges: stream s owns rows [begins[s], ends[s]).
+int stream_signature_ranges(void* out, const void* increments, const int64_t* begins,
+                            const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
+                            rpy_dtype dtype, void* stream);
+
 // Prefix length P the specialised kernel uses for (width, depth, dtype), or
 // 0 when the shape runs on the generic kernel.
 int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64);
@@ -18,5 +23,21 @@ int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64);
 // Host run of the lane decomposition for one double-precision stream.
 int signature_host_emulation(double* out, const double* incr, int64_t len, int32_t width,
                              int32_t depth, int32_t prefix);
+
+}  // namespace rpy_amd
+
+namespace rpy_amd {
+
+// dyadic.hip
+int64_t dyadic_cache_elements(int64_t n_streams, int32_t width, int32_t depth, int32_t levels);
+int dyadic_cache_build(void* cache, const void* increments, const double* params,
+                       const int64_t* offsets, int64_t n_streams, int32_t width, int32_t depth,
+                       int32_t resolution, int32_t levels, rpy_dtype dtype, void* stream);
+int dyadic_cache_query(void* out, const void* cache, const void* increments, const double* params,
+                       const int64_t* offsets, int64_t n_streams, int32_t width, int32_t depth,
+                       int32_t resolution, int32_t levels, const int64_t* q_stream,
+                       const double* q_inf, const double* q_sup, int64_t n_queries,
+                       rpy_dtype dtype, void* stream);
+int dyadic_cover_host(int64_t A, int64_t B, int r, int64_t* ks, int* ns, int max_pieces);
 
 }  // namespace rpy_amd

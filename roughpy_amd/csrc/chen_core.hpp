@@ -18,6 +18,13 @@
 // P_K of a prefix is shared by all W^(M-K) words below it, which makes the
 // total work sum_K (D - K + 1) W^K fused multiply-adds per step -- the same
 // count as the reference's truncated in-place Horner.
+//
+// The scaled increments x_a / d come through a row accessor: ScaledRow reads
+// them from a table of D*W values built once per increment and shared by all
+// lanes of a stream (row[(d-1)*W + a] = x_a * (1/d)), so a step issues only
+// fused multiply-adds; RawRow reads the W raw values (and the lane's path
+// letters) once and scales in registers, which moves 3x fewer bytes from
+// LDS for wide float64 lanes.
 #pragma once
 
 #include <cmath>
@@ -56,6 +63,50 @@ struct Shape {
         return s;
     }
     static constexpr int SUB = off(D + 1);
+    static constexpr int ROW = D * W;  // scaled-increment table per step
+};
+
+// Fills row[(d-1)*W + a] = x[a] * (1/d) for d = 1..D.
+template <typename T, int W, int D>
+RPY_HD inline void build_row(T* row, const T* x)
+{
+    for (int d = 1; d <= D; ++d)
+        for (int a = 0; a < W; ++a) row[(d - 1) * W + a] = (d == 1) ? x[a] : x[a] * (T(1) / T(d));
+}
+
+// x_a / d read from the scaled table.
+template <typename T, int W, int D, int P>
+struct ScaledRow {
+    const T* row;
+    const int (&u)[P];
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T x(int a) const { return row[(d - 1) * W + a]; }
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T xu(int j) const { return row[(d - 1) * W + u[j]]; }
+};
+
+// x_a / d computed from the raw increment held in registers.
+template <typename T, int W, int D, int P>
+struct RawRow {
+    T raw[W];
+    T rawu[P];
+    RPY_HD inline __attribute__((always_inline)) RawRow(const T* row, const int (&u)[P])
+    {
+        for (int a = 0; a < W; ++a) raw[a] = row[a];
+        for (int j = 0; j < P; ++j) rawu[j] = row[u[j]];
+    }
+    template <int d>
+    static RPY_HD inline __attribute__((always_inline)) T sc(T v)
+    {
+        if constexpr (d == 1)
+            return v;
+        else
+            return v * (T(1) / T(d));
+    }
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T x(int a) const { return sc<d>(raw[a]); }
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T xu(int j) const { return sc<d>(rawu[j]); }
 };
 
 template <typename T, int W, int D, int P>
@@ -71,99 +122,88 @@ struct Lane {
         static_for<S::SUB>([&](auto i) { sub[i] = T(0); });
     }
 
-    // x[a]: increment; xu[j] = x[u_j] (the lane's path letters).
-    RPY_HD inline __attribute__((always_inline)) void step(const T (&x)[W], const T (&xu)[P])
+    // One Chen step.  acc: row accessor (ScaledRow or RawRow) giving x_a / d
+    // and x_{u_j} / d for the lane's path letters u_j.
+    template <typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void step(const Acc& acc)
     {
         // subtree targets, top-down
         static_for<D - P + 1>([&](auto i) {
             constexpr int M = D - int(decltype(i)::value);
-            target<M>(x, xu);
+            target<M>(acc);
         });
         // path targets M = P-1 .. 1
         static_for<P - 1>([&](auto i) {
             constexpr int M = P - 1 - int(decltype(i)::value);
-            path_target<M>(xu);
+            path_target<M>(acc);
         });
     }
 
   private:
-    template <int d>
-    static RPY_HD inline __attribute__((always_inline)) T scaled(T v)
-    {
-        if constexpr (d == 1)
-            return v;
-        else
-            return v * (T(1) / T(d));
-    }
 
     static RPY_HD inline __attribute__((always_inline)) T fmaT(T a, T b, T c)
     {
-#ifdef __HIP_DEVICE_COMPILE__
-        return __builtin_fma(a, b, c);
-#else
-        return std::fma(a, b, c);
-#endif
+        // type-exact fused multiply-add (no promotion of float to double)
+        if constexpr (std::is_same<T, float>::value)
+            return __builtin_fmaf(a, b, c);
+        else
+            return __builtin_fma(a, b, c);
     }
 
-    template <int M>
-    RPY_HD inline __attribute__((always_inline)) T path_partial(const T (&xu)[P]) const
+    template <int M, typename Acc>
+    RPY_HD inline __attribute__((always_inline)) T path_partial(const Acc& acc) const
     {
         // P_{P-1} along the lane's prefix, for target M
         T p = path[0];
         static_for<P - 1>([&](auto k0) {
             constexpr int K = int(decltype(k0)::value) + 1;
-            p = fmaT(p, scaled<M - K + 1>(xu[K - 1]), path[K]);
+            p = fmaT(p, acc.template xu<M - K + 1>(K - 1), path[K]);
         });
         return p;
     }
 
-    template <int M>
-    RPY_HD inline __attribute__((always_inline)) void path_target(const T (&xu)[P])
+    template <int M, typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void path_target(const Acc& acc)
     {
         // S_M(u_0..u_{M-1}) for M < P
         T p = path[0];
         static_for<M - 1>([&](auto k0) {
             constexpr int K = int(decltype(k0)::value) + 1;
-            p = fmaT(p, scaled<M - K + 1>(xu[K - 1]), path[K]);
+            p = fmaT(p, acc.template xu<M - K + 1>(K - 1), path[K]);
         });
-        path[M] = fmaT(p, xu[M - 1], path[M]);
+        path[M] = fmaT(p, acc.template xu<1>(M - 1), path[M]);
     }
 
-    template <int M>
-    RPY_HD inline __attribute__((always_inline)) void target(const T (&x)[W], const T (&xu)[P])
+    template <int M, typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void target(const Acc& acc)
     {
-        const T p = path_partial<M>(xu);
+        const T p = path_partial<M>(acc);
         // K = P: the single word u
-        const T q0 = fmaT(p, scaled<M - P + 1>(xu[P - 1]), sub[0]);
+        const T q0 = fmaT(p, acc.template xu<M - P + 1>(P - 1), sub[0]);
         if constexpr (M == P) {
             sub[0] = q0;
         } else {
-            T q[1] = {q0};
-            stage<P + 1, M, 1>(q, x);
+            stage<P + 1, M, 0>(q0, acc);
         }
     }
 
-    // q_K[v] = sub_K[v] + q_{K-1}[v / W] * x[v % W] / (M - K + 1)
-    template <int K, int M, int N>
-    RPY_HD inline __attribute__((always_inline)) void stage(const T (&qprev)[N], const T (&x)[W])
+    // Depth-first over the subtree below word V (level K-1 of the subtree):
+    //   q_K[V W + a] = sub_K[V W + a] + q_{K-1}[V] x[a] / (M - K + 1).
+    // Only one partial per level is live at a time, which keeps the register
+    // footprint at the lane state plus O(W) temporaries.
+    template <int K, int M, int V, typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void stage(T qprev, const Acc& acc)
     {
-        constexpr int NK = int(ipow(W, K - P));
         constexpr int O = S::off(K);
-        T xs[W];
-        static_for<W>([&](auto a) { xs[a] = scaled<M - K + 1>(x[a]); });
-        if constexpr (K == M) {
-            static_for<NK>([&](auto v) {
-                constexpr int vi = int(decltype(v)::value);
-                sub[O + vi] = fmaT(qprev[vi / W], xs[vi % W], sub[O + vi]);
-            });
-        } else {
-            T q[NK];
-            static_for<NK>([&](auto v) {
-                constexpr int vi = int(decltype(v)::value);
-                q[vi] = fmaT(qprev[vi / W], xs[vi % W], sub[O + vi]);
-            });
-            stage<K + 1, M, NK>(q, x);
-        }
+        static_for<W>([&](auto a) {
+            constexpr int ai = int(decltype(a)::value);
+            constexpr int v = V * W + ai;
+            const T q = fmaT(qprev, acc.template x<M - K + 1>(ai), sub[O + v]);
+            if constexpr (K == M)
+                sub[O + v] = q;
+            else
+                stage<K + 1, M, v>(q, acc);
+        });
     }
 };
 

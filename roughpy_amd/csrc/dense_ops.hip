@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "dense_ops.hpp"
+#include "kernel_util.hpp"
 #include "rpy_common.hpp"
 
 namespace rpy_amd {
@@ -34,41 +35,6 @@ constexpr size_t kLdsBudget = 64 * 1024;
 struct Bounds {
     int32_t lo, hi;
 };
-
-// Exact unsigned division by a runtime divisor d >= 1 for n < 2^31
-// (Granlund-Montgomery round-up method: q = (umulhi(m, n) + n) >> s).
-struct FastDiv {
-    uint32_t d, m, s;
-};
-
-FastDiv make_fastdiv(uint32_t d)
-{
-    uint32_t s = 0;
-    while ((uint64_t(1) << s) < d) ++s;
-    uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1;
-    return FastDiv{d, uint32_t(m), s};
-}
-
-RPY_FI uint32_t fdiv(uint32_t n, const FastDiv& f)
-{
-    return (__umulhi(f.m, n) + n) >> f.s;
-}
-
-// Per-basis powers of the width as fast divisors: pw[k] divides by w^k.
-struct Pow {
-    FastDiv pw[kMaxDepth + 1];
-};
-
-Pow make_pow(int w, int D)
-{
-    Pow p{};
-    uint64_t v = 1;
-    for (int k = 0; k <= D && k <= kMaxDepth; ++k) {
-        p.pw[k] = make_fastdiv(uint32_t(v));
-        v *= uint64_t(w);
-    }
-    return p;
-}
 
 int grid_for(int64_t batch)
 {

@@ -1,0 +1,347 @@
+// dyadic.hip -- dyadic signature cache over [0, 1) and interval queries.
+//
+// Semantics (the reference's, checked against the oracle): a query [inf, sup)
+// at resolution r is dissected into maximal aligned dyadic pieces
+// (intervals/src/dyadic_interval.cpp:191-228) and its signature is the Chen
+// product, left to right, of the rows whose parameter lies in each piece --
+// what Stream::signature returns for a LieIncrementStream of resolution r
+// (streams/src/stream.cpp:107-171, streams/src/dyadic_caching_layer.cpp:
+// 86-130).  The reference memoises only resolution-r leaves and recomputes
+// every coarser node per query; here every node of power 0..J (a node of
+// power j covers 2^(r-j) leaves) is built once, so a query costs at most
+// 2(J+1) node products plus at most 2(2^(r-J) - 1) Chen steps from raw rows.
+//
+// Cache layout (level-major, so level J is one contiguous batch the
+// signature kernel writes directly, and level j-1 is one strided batch of
+// products of level j):
+//   node(j, s, k) at element offset ((2^j - 1) * n_streams + s * 2^j + k) * tsize.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "chen.hpp"
+#include "dense_ops.hpp"
+#include "kernel_util.hpp"
+#include "rpy_common.hpp"
+
+namespace rpy_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxRes = 30;
+constexpr int kMaxPieces = 2 * kMaxRes + 2;
+
+RPY_HD inline int64_t node_offset(int j, int64_t n_streams, int64_t s, int64_t k)
+{
+    return ((int64_t(1) << j) - 1) * n_streams + s * (int64_t(1) << j) + k;
+}
+
+// first index in [lo, hi) with params[i] >= v
+RPY_FI int64_t lower_bound_dev(const double* params, int64_t lo, int64_t hi, double v)
+{
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (params[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// begins/ends of the level-J segments: rows of stream s with params in
+// [k 2^-J, (k+1) 2^-J).
+__global__ void segment_bounds_kernel(const double* __restrict__ params,
+                                      const int64_t* __restrict__ offsets, int64_t n_streams,
+                                      int J, int64_t* __restrict__ begins,
+                                      int64_t* __restrict__ ends)
+{
+    const int64_t nseg = n_streams << J;
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nseg;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = t >> J;
+        const int64_t k = t - (s << J);
+        const int64_t lo = offsets[s], hi = offsets[s + 1];
+        begins[t] = lower_bound_dev(params, lo, hi, std::ldexp(double(k), -J));
+        ends[t] = lower_bound_dev(params, lo, hi, std::ldexp(double(k + 1), -J));
+    }
+}
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = T(0);
+}
+
+// Maximal aligned dyadic pieces of [A, B) (units of 2^-r, 0 <= A <= B <=
+// 2^r), left to right: at each step the largest aligned block that starts at
+// A and ends by B.  Returns the count; piece i is (ks[i], ns[i]).
+RPY_HD inline int dyadic_cover(int64_t A, int64_t B, int r, int64_t* ks, int* ns)
+{
+    int n = 0;
+    while (A < B) {
+        int lg = r;
+        if (A != 0) {
+            lg = 0;
+            while (((A >> lg) & 1) == 0) ++lg;
+        }
+        while (A + (int64_t(1) << lg) > B) --lg;
+        ns[n] = r - lg;
+        ks[n] = A >> lg;
+        ++n;
+        A += int64_t(1) << lg;
+    }
+    return n;
+}
+
+// S <- S (x) N, both in LDS, top-down with a barrier per level.
+template <typename T>
+RPY_FI void wg_mul_inplace(T* S, const T* N, const DevBasis& b, const Pow& pw)
+{
+    for (int m = b.depth; m >= 1; --m) {
+        const uint32_t n = uint32_t(b.level_size(m));
+        for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+            T acc = T(0);
+            for (int i = 0; i <= m; ++i) {  // lhs degree m-i, high to low
+                const uint32_t pre = fdiv(idx, pw.pw[i]);
+                const uint32_t suf = idx - pre * pw.pw[i].d;
+                acc += S[b.db[m - i] + pre] * N[b.db[i] + suf];
+            }
+            S[b.db[m] + idx] = acc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[0] = S[0] * N[0];
+    __syncthreads();
+}
+
+// S <- S (x) exp(x) for a degree-one increment x, the Horner of the
+// signature kernel evaluated per coefficient (top-down, barrier per level).
+template <typename T>
+RPY_FI void wg_chen_step(T* S, const T* x, const T* inv, const DevBasis& b, const Pow& pw)
+{
+    const uint32_t W = uint32_t(b.width);
+    for (int m = b.depth; m >= 1; --m) {
+        const uint32_t n = uint32_t(b.level_size(m));
+        for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+            T p = S[0];
+            for (int K = 1; K <= m; ++K) {
+                const uint32_t pre = fdiv(idx, pw.pw[m - K]);       // prefix of length K
+                const uint32_t letter = pre - fdiv(pre, pw.pw[1]) * W;
+                p = p * (x[letter] * inv[m - K + 1]) + S[b.db[K] + pre];
+            }
+            S[b.db[m] + idx] = p;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dyadic_query_kernel(
+        T* __restrict__ out, const T* __restrict__ cache, const T* __restrict__ incr,
+        const double* __restrict__ params, const int64_t* __restrict__ offsets, int64_t n_streams,
+        DevBasis b, Pow pw, int r, int J, const int64_t* __restrict__ q_stream,
+        const double* __restrict__ q_inf, const double* __restrict__ q_sup, int64_t n_queries)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t tsize = b.size();
+    T* S = reinterpret_cast<T*>(smem);
+    T* N = S + tsize;
+    __shared__ int64_t pk[kMaxPieces];
+    __shared__ int pn[kMaxPieces];
+    __shared__ int npieces;
+    __shared__ int64_t row_lo, row_hi;
+    __shared__ T xrow[32];
+    __shared__ T inv[kMaxDepth + 2];
+    const int W = b.width;
+    if (threadIdx.x <= kMaxDepth + 1) inv[threadIdx.x] = threadIdx.x ? T(1) / T(threadIdx.x) : T(0);
+
+    for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
+        const int64_t s = q_stream[q];
+        const bool valid = s >= 0 && s < n_streams;
+        if (threadIdx.x == 0) {
+            int cnt = 0;
+            if (valid) {
+                const double scale = std::ldexp(1.0, r);
+                const double full = scale;
+                double a = std::floor(q_inf[q] * scale), c = std::floor(q_sup[q] * scale);
+                a = std::min(std::max(a, 0.0), full);
+                c = std::min(std::max(c, 0.0), full);
+                if (a < c) cnt = dyadic_cover(int64_t(a), int64_t(c), r, pk, pn);
+            }
+            npieces = cnt;
+        }
+        for (int64_t i = threadIdx.x; i < tsize; i += kBlock) S[i] = (i == 0) ? T(1) : T(0);
+        __syncthreads();
+        const int np = npieces;
+        for (int p = 0; p < np; ++p) {
+            const int j = pn[p];
+            const int64_t k = pk[p];
+            if (j <= J) {
+                const T* node = cache + node_offset(j, n_streams, s, k) * tsize;
+                for (int64_t i = threadIdx.x; i < tsize; i += kBlock) N[i] = node[i];
+                __syncthreads();
+                wg_mul_inplace(S, N, b, pw);
+            } else {
+                if (threadIdx.x == 0) {
+                    const int64_t lo = offsets[s], hi = offsets[s + 1];
+                    row_lo = lower_bound_dev(params, lo, hi, std::ldexp(double(k), -j));
+                    row_hi = lower_bound_dev(params, lo, hi, std::ldexp(double(k + 1), -j));
+                }
+                __syncthreads();
+                const int64_t rlo = row_lo, rhi = row_hi;
+                __syncthreads();  // every wave has read the bounds before they change
+                for (int64_t row = rlo; row < rhi; ++row) {
+                    if (threadIdx.x < W) xrow[threadIdx.x] = incr[row * W + threadIdx.x];
+                    __syncthreads();
+                    wg_chen_step(S, xrow, inv, b, pw);
+                }
+            }
+        }
+        T* o = out + q * tsize;
+        for (int64_t i = threadIdx.x; i < tsize; i += kBlock) o[i] = S[i];
+        __syncthreads();
+    }
+}
+
+uint32_t blocks_for(int64_t n)
+{
+    return uint32_t(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65536)));
+}
+
+int check_args(int32_t width, int32_t depth, int32_t resolution, int32_t levels, rpy_dtype dtype,
+               DevBasis& b)
+{
+    rpy_tensor_basis tb{width, depth, nullptr};
+    int rc = make_basis(&tb, b);
+    if (rc) return rc;
+    if (depth < 1) return fail(RPY_ERR_INVALID_ARGUMENT, "the cache needs depth >= 1");
+    if (width > 32) return fail(RPY_ERR_UNSUPPORTED, "the dyadic cache supports width <= 32");
+    if (dtype != RPY_F32 && dtype != RPY_F64) return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    if (resolution < 0 || resolution > kMaxRes)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "resolution must be in [0, %d]", kMaxRes);
+    if (levels < 0 || levels > resolution || levels > 24)
+        return fail(RPY_ERR_INVALID_ARGUMENT,
+                    "stored levels must be in [0, min(resolution, 24)], got %d", levels);
+    return RPY_OK;
+}
+
+}  // namespace
+
+int64_t dyadic_cache_elements(int64_t n_streams, int32_t width, int32_t depth, int32_t levels)
+{
+    const int64_t tsize = tensor_size_checked(width, depth);
+    if (tsize < 0 || n_streams < 0 || levels < 0 || levels > 24) return -1;
+    return ((int64_t(1) << (levels + 1)) - 1) * n_streams * tsize;
+}
+
+int dyadic_cache_build(void* cache, const void* increments, const double* params,
+                       const int64_t* offsets, int64_t n_streams, int32_t width, int32_t depth,
+                       int32_t resolution, int32_t levels, rpy_dtype dtype, void* stream)
+{
+    DevBasis b;
+    int rc = check_args(width, depth, resolution, levels, dtype, b);
+    if (rc) return rc;
+    if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
+    if (n_streams == 0) return RPY_OK;
+    if (!cache || !increments || !params || !offsets)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int J = levels;
+    const int64_t tsize = b.size();
+    const int64_t nseg = n_streams << J;
+    const size_t es = dtype == RPY_F64 ? 8 : 4;
+
+    int64_t* bounds = nullptr;
+    RPY_HIP_CHECK(hipMallocAsync((void**)&bounds, size_t(2 * nseg) * sizeof(int64_t), st));
+    hipLaunchKernelGGL(segment_bounds_kernel, dim3(blocks_for(nseg)), dim3(kBlock), 0, st, params,
+                       offsets, n_streams, J, bounds, bounds + nseg);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "segment launch failed: %s", hipGetErrorString(err));
+    // level J: one signature per segment, written in place
+    char* base = static_cast<char*>(cache);
+    if (!rc)
+        rc = stream_signature_ranges(base + size_t(node_offset(J, n_streams, 0, 0) * tsize) * es,
+                                     increments, bounds, bounds + nseg, nseg, width, depth, dtype,
+                                     stream);
+    // levels J-1 .. 0: node(j, s, k) = node(j+1, s, 2k) (x) node(j+1, s, 2k+1)
+    rpy_tensor_basis tb{width, depth, nullptr};
+    for (int j = J - 1; j >= 0 && !rc; --j) {
+        const int64_t batch = n_streams << j;
+        char* outp = base + size_t(node_offset(j, n_streams, 0, 0) * tsize) * es;
+        const char* lhs = base + size_t(node_offset(j + 1, n_streams, 0, 0) * tsize) * es;
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(blocks_for(batch * tsize)), dim3(kBlock),
+                               0, st, (double*)outp, batch * tsize);
+        else
+            hipLaunchKernelGGL(fill_zero_kernel<float>, dim3(blocks_for(batch * tsize)), dim3(kBlock),
+                               0, st, (float*)outp, batch * tsize);
+        err = hipGetLastError();
+        if (err != hipSuccess) {
+            rc = fail(RPY_ERR_HIP, "zero launch failed: %s", hipGetErrorString(err));
+            break;
+        }
+        rc = dense_ft_fma(outp, lhs, lhs + size_t(tsize) * es, dtype, &tb, {0, -1}, {0, -1}, {0, -1},
+                          batch, tsize, 2 * tsize, 2 * tsize, stream);
+    }
+    hipError_t ferr = hipFreeAsync(bounds, st);
+    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    return rc;
+}
+
+int dyadic_cache_query(void* out, const void* cache, const void* increments, const double* params,
+                       const int64_t* offsets, int64_t n_streams, int32_t width, int32_t depth,
+                       int32_t resolution, int32_t levels, const int64_t* q_stream,
+                       const double* q_inf, const double* q_sup, int64_t n_queries,
+                       rpy_dtype dtype, void* stream)
+{
+    DevBasis b;
+    int rc = check_args(width, depth, resolution, levels, dtype, b);
+    if (rc) return rc;
+    if (n_queries < 0 || n_streams < 0) return fail(RPY_ERR_SHAPE, "negative size");
+    if (n_queries == 0) return RPY_OK;
+    if (!out || !cache || !increments || !params || !offsets || !q_stream || !q_inf || !q_sup)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    const int64_t tsize = b.size();
+    const size_t es = dtype == RPY_F64 ? 8 : 4;
+    const size_t lds = 2 * size_t(tsize) * es;
+    if (lds > 64 * 1024)
+        return fail(RPY_ERR_UNSUPPORTED, "queries need %zu bytes of LDS (limit 65536)", lds);
+    const Pow pw = make_pow(width, depth);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const uint32_t grid = uint32_t(std::min<int64_t>(n_queries, 16384));
+    if (dtype == RPY_F64)
+        hipLaunchKernelGGL(dyadic_query_kernel<double>, dim3(grid), dim3(kBlock), lds, st,
+                           (double*)out, (const double*)cache, (const double*)increments, params,
+                           offsets, n_streams, b, pw, resolution, levels, q_stream, q_inf, q_sup,
+                           n_queries);
+    else
+        hipLaunchKernelGGL(dyadic_query_kernel<float>, dim3(grid), dim3(kBlock), lds, st,
+                           (float*)out, (const float*)cache, (const float*)increments, params,
+                           offsets, n_streams, b, pw, resolution, levels, q_stream, q_inf, q_sup,
+                           n_queries);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+// Host mirror of the device cover, for CPU tests of the dissection.
+int dyadic_cover_host(int64_t A, int64_t B, int r, int64_t* ks, int* ns, int max_pieces)
+{
+    int64_t k[kMaxPieces];
+    int n[kMaxPieces];
+    if (r < 0 || r > kMaxRes || A < 0 || B > (int64_t(1) << r))
+        return -1;
+    const int cnt = dyadic_cover(A, B, r, k, n);
+    for (int i = 0; i < cnt && i < max_pieces; ++i) {
+        ks[i] = k[i];
+        ns[i] = n[i];
+    }
+    return cnt;
+}
+
+}  // namespace rpy_amd

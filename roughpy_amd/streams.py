@@ -25,7 +25,8 @@ from .bases import LieBasis, TensorBasis
 from .compute import DenseFreeTensor, DenseLie
 
 __all__ = [
-    "RealInterval", "LieIncrementStream", "signature_batch", "log_signature_batch",
+    "RealInterval", "LieIncrementStream", "DyadicSignatureCache", "signature_batch",
+    "log_signature_batch",
     "param_to_resolution", "to_dyadic_intervals",
 ]
 
@@ -137,6 +138,75 @@ def log_signature_batch(increments, offsets, width: int, depth: int, dtype=None,
         out.data_ptr(), inc.data_ptr(), off.data_ptr(), n, int(width), int(depth), code,
         t2l.device_struct(), _device.stream_handle()))
     return _device.to_host(out) if isinstance(increments, np.ndarray) else out
+
+
+class DyadicSignatureCache:
+    """Dyadic signature cache of many streams over [0, 1), built on the GPU.
+
+    increments: (rows, width); params: (rows,) float64, strictly increasing
+    within each stream and in [0, 1); offsets: (n_streams + 1,) CSR offsets.
+    Nodes of powers 0..levels are stored (level-major); ``query`` answers
+    clopen interval queries at ``resolution`` with the semantics of
+    Stream.signature(RealInterval(inf, sup), resolution) for streams of that
+    resolution (streams/src/stream.cpp:107-171).
+    """
+
+    def __init__(self, increments, params, offsets, width: int, depth: int,
+                 resolution: int = 12, levels: int = 6, dtype=None):
+        dt = np.dtype(dtype) if dtype is not None else _device.np_dtype(increments)
+        if dt not in (np.float32, np.float64):
+            dt = np.dtype(np.float64)
+        self.dtype = dt
+        self.width, self.depth = int(width), int(depth)
+        self.resolution, self.levels = int(resolution), int(levels)
+        t = _device.torch()
+        self._inc = _device.to_device(increments, dt).reshape(-1, self.width)
+        self._params = _device.to_device(params).to(t.float64).contiguous()
+        self._offs = _device.to_device(np.asarray(offsets, dtype=np.int64)
+                                       if isinstance(offsets, (list, tuple, np.ndarray))
+                                       else offsets).to(t.int64).contiguous()
+        self.n_streams = int(self._offs.shape[0]) - 1
+        L = _capi.lib()
+        n = int(L.rpy_dyadic_cache_elements(self.n_streams, self.width, self.depth, self.levels))
+        if n < 0:
+            raise ValueError("invalid cache shape")
+        self.tsize = int(L.rpy_tensor_size(self.width, self.depth))
+        self.cache = _device.empty((n,), dt)
+        self.build()
+
+    def build(self):
+        _capi.check(_capi.lib().rpy_dyadic_cache_build(
+            self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
+            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self.resolution,
+            self.levels, _dtype_code(self.dtype), _device.stream_handle()))
+
+    def node(self, power: int, stream: int, k: int):
+        """Stored signature of stream `stream` over [k 2^-power, (k+1) 2^-power)."""
+        if not (0 <= power <= self.levels and 0 <= k < (1 << power)):
+            raise IndexError("node outside the stored levels")
+        off = ((1 << power) - 1) * self.n_streams + stream * (1 << power) + k
+        return self.cache[off * self.tsize:(off + 1) * self.tsize]
+
+    def query(self, stream_ids, inf, sup, out=None):
+        t = _device.torch()
+        sid = _device.to_device(np.asarray(stream_ids, dtype=np.int64)
+                                if not _device.is_device_array(stream_ids) else stream_ids)
+        sid = sid.to(t.int64).contiguous()
+        qi = _device.to_device(np.asarray(inf, dtype=np.float64)
+                               if not _device.is_device_array(inf) else inf).to(t.float64).contiguous()
+        qs = _device.to_device(np.asarray(sup, dtype=np.float64)
+                               if not _device.is_device_array(sup) else sup).to(t.float64).contiguous()
+        nq = int(sid.shape[0])
+        if qi.shape[0] != nq or qs.shape[0] != nq:
+            raise ValueError("stream_ids, inf and sup must have the same length")
+        if out is None:
+            out = _device.empty((nq, self.tsize), self.dtype)
+        _capi.check(_capi.lib().rpy_dyadic_cache_query(
+            out.data_ptr(), self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
+            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self.resolution,
+            self.levels, sid.data_ptr(), qi.data_ptr(), qs.data_ptr(), nq,
+            _dtype_code(self.dtype), _device.stream_handle()))
+        return out
 
 
 class LieIncrementStream:

@@ -235,3 +235,20 @@ def test_error_codes_map_to_exceptions():
                             _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
                             _capi.DegreeBoundsC(0, -1), 0, 7, 7, 7, None)
     assert rc == _capi.RPY_OK
+
+
+def test_query_kernel_dyadic_cover_matches_reference_dissection(rng):
+    """The query kernel's greedy cover of [A, B) (leaf units) equals
+    to_dyadic_intervals of the clopen [A 2^-r, B 2^-r) (oracle restatement
+    of intervals/src/dyadic_interval.cpp:191-228)."""
+    L = _capi.lib()
+    ks = np.zeros(64, dtype=np.int64)
+    ns = np.zeros(64, dtype=np.int32)
+    for r in (0, 1, 3, 7, 12):
+        for _ in range(200):
+            a, b = sorted(rng.integers(0, 2 ** r + 1, size=2))
+            n = L.rpy_debug_dyadic_cover(int(a), int(b), r, _capi.i64p(ks),
+                                         ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 64)
+            got = [(int(ks[i]), int(ns[i])) for i in range(n)]
+            want = O.to_dyadic_intervals(a / 2.0 ** r, b / 2.0 ** r, r) if a < b else []
+            assert got == want, (a, b, r)
