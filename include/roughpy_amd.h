@@ -261,6 +261,18 @@ int rpy_dyadic_cache_query(void *out, const void *cache, const void *increments,
                            const double *query_sup, int64_t n_queries,
                            rpy_dtype dtype, void *stream);
 
+/* ---- signature kernel by signature dot (device) ------------------------- */
+
+/* gram[i, j] = <S_i, S_j> for the n dense signatures S (n x k float32, row
+ * stride ld_signatures), the signature kernel of
+ * examples/signature-kernel-by-signature-dot.py:15-23 (np.inner of dense
+ * signatures) for a whole batch: one GEMM on MFMA with each float32 split
+ * into bf16 hi + lo parts (hi.hi + hi.lo + lo.hi, float32 accumulate).
+ * gram: n x n float32, row stride ld_gram.  All pointers device. */
+int rpy_signature_gram(float *gram, const float *signatures, int64_t n,
+                       int64_t k, int64_t ld_signatures, int64_t ld_gram,
+                       void *stream);
+
 /* ---- stream queries (host) ---------------------------------------------- */
 
 /* -min(0, e-1) where arg = m 2^e, 0.5 <= |m| < 1; mirror of

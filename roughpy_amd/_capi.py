@@ -101,6 +101,7 @@ SIGNATURES = {
     "rpy_dyadic_cache_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32,
                                               _i32, _vp, _vp, _vp, _i64, ctypes.c_int, _vp]),
     "rpy_debug_dyadic_cover": (_i32, [_i64, _i64, _i32, _p_i64, ctypes.POINTER(_i32), _i32]),
+    "rpy_signature_gram": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp]),
 }
 
 

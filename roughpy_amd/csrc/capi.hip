@@ -307,6 +307,12 @@ int rpy_dyadic_cache_query(void* out, const void* cache, const void* increments,
                               dtype, stream);
 }
 
+int rpy_signature_gram(float* gram, const float* signatures, int64_t n, int64_t k,
+                       int64_t ld_signatures, int64_t ld_gram, void* stream)
+{
+    return gram_bf16x3(gram, signatures, n, k, ld_signatures, ld_gram, stream);
+}
+
 int32_t rpy_debug_dyadic_cover(int64_t begin, int64_t end, int32_t resolution, int64_t* k_out,
                                int32_t* n_out, int32_t max_pieces)
 {

@@ -40,4 +40,8 @@ int dyadic_cache_query(void* out, const void* cache, const void* increments, con
                        rpy_dtype dtype, void* stream);
 int dyadic_cover_host(int64_t A, int64_t B, int r, int64_t* ks, int* ns, int max_pieces);
 
+// gram.hip
+int gram_bf16x3(float* G, const float* S, int64_t n, int64_t k, int64_t lds, int64_t ldg,
+                void* stream);
+
 }  // namespace rpy_amd

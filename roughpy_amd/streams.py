@@ -140,6 +140,24 @@ def log_signature_batch(increments, offsets, width: int, depth: int, dtype=None,
     return _device.to_host(out) if isinstance(increments, np.ndarray) else out
 
 
+def signature_gram(signatures):
+    """Gram matrix of dense signatures, G[i, j] = <S_i, S_j> (float32).
+
+    The signature kernel by signature dot
+    (examples/signature-kernel-by-signature-dot.py:15-23) for a batch: one
+    MFMA GEMM with bf16 hi/lo split operands and float32 accumulation.
+    signatures: (n, k) float32 (NumPy or GPU tensor).
+    """
+    S = _device.to_device(signatures, np.float32)
+    if S.ndim != 2:
+        raise ValueError("signatures must be a 2-d array (n, tensor_size)")
+    n, k = int(S.shape[0]), int(S.shape[1])
+    G = _device.empty((n, n), np.float32)
+    _capi.check(_capi.lib().rpy_signature_gram(G.data_ptr(), S.data_ptr(), n, k, k, n,
+                                               _device.stream_handle()))
+    return _device.to_host(G) if isinstance(signatures, np.ndarray) else G
+
+
 class DyadicSignatureCache:
     """Dyadic signature cache of many streams over [0, 1), built on the GPU.
 
