@@ -261,6 +261,33 @@ int rpy_dyadic_cache_query(void *out, const void *cache, const void *increments,
                            const double *query_sup, int64_t n_queries,
                            rpy_dtype dtype, void *stream);
 
+/* ---- derivatives (device) ----------------------------------------------- */
+
+/* Vector-Jacobian product of rpy_stream_signature_batch: grad_increments
+ * (same shape as increments) receives d<G_s, S_s>/dx for every row, where
+ * S_s = signatures[s] (the forward outputs) and G_s = grad_signatures[s].
+ * The reference only has the forward-mode derivative
+ * (algebra/src/lite_context.h:580-636); this is its reverse mode, by reverse
+ * Chen recomputation (prefixes rebuilt by multiplying with exp(-x), nothing
+ * stored per step).  dtype RPY_F32 or RPY_F64; width <= 32. */
+int rpy_stream_signature_vjp(void *grad_increments, const void *increments,
+                             const int64_t *offsets, int64_t n_streams,
+                             int32_t width, int32_t depth,
+                             const void *signatures,
+                             const void *grad_signatures, rpy_dtype dtype,
+                             void *stream);
+
+/* out += R_op^*(arg), the adjoint of right multiplication by op:
+ * out[u] += sum_v arg[u v] op[v]; right-hand counterpart of
+ * ft_adj_lmul (roughpy_compute/dense/basic/free_tensor_adjoint_left_mul.hpp:
+ * 10-77), used by the backward of the tensor product. */
+int rpy_dense_ft_adjoint_right_mul(void *out, const void *op, const void *arg,
+                                   rpy_dtype dtype,
+                                   const rpy_tensor_basis *basis,
+                                   int64_t batch, int64_t out_stride,
+                                   int64_t op_stride, int64_t arg_stride,
+                                   void *stream);
+
 /* ---- signature kernel by signature dot (device) ------------------------- */
 
 /* gram[i, j] = <S_i, S_j> for the n dense signatures S (n x k float32, row

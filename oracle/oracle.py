@@ -18,8 +18,9 @@ Contents, each restating the reference algorithm it cites:
 * the dyadic interval dissection and the stream query semantics of
   LieIncrementStream / DyadicCachingLayer.
 
-Parity pins (what checks each function beyond retyping it) are listed in
-oracle/README.md and exercised by tests/test_oracle_*.py.
+The parity pins of every function (reference goldens, the reference
+kernels' own vectors, closed forms, invariants, exact rational brute force)
+are exercised by tests/test_oracle_pins.py.
 """
 from __future__ import annotations
 
@@ -609,3 +610,34 @@ class IncrementStream:
     def log_signature(self, inf: float, sup: float, resolution: Optional[int] = None) -> np.ndarray:
         s = self.signature(inf, sup, resolution)
         return tensor_to_lie(ft_log(s, self.width, self.depth), self.width, self.depth)
+
+
+# ---------------------------------------------------------------------------
+# Signature derivative (forward mode)
+# ---------------------------------------------------------------------------
+
+def dexp_series(X: np.ndarray, P: np.ndarray, width: int, depth: int) -> np.ndarray:
+    """sum_{d>=0} (-1)^d ad_X^d(P) / (d+1)!, restating derive_series_compute
+    (algebra/src/lite_context.h:580-601)."""
+    def ad(a):
+        return ft_mul(X, a, width, depth) - ft_mul(a, X, width, depth)
+    result = P.copy()
+    term = ad(P)
+    fact = 1.0
+    for d in range(1, depth + 1):
+        fact *= d + 1
+        result = result + term * ((-1.0) ** d / fact)
+        term = ad(term)
+    return result
+
+
+def signature_derivative(pairs, width: int, depth: int) -> np.ndarray:
+    """pairs: [(E_i, P_i)] with E_i the interval signatures and P_i the
+    perturbations as tensors.  Restates sig_derivative_impl
+    (algebra/src/lite_context.h:616-636)."""
+    result = np.zeros(tensor_size(width, depth))
+    for E, P in pairs:
+        X = ft_log(E, width, depth)
+        result = ft_mul(result, E, width, depth) + ft_mul(E, dexp_series(X, P, width, depth),
+                                                          width, depth)
+    return result

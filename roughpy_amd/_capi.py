@@ -102,6 +102,10 @@ SIGNATURES = {
                                               _i32, _vp, _vp, _vp, _i64, ctypes.c_int, _vp]),
     "rpy_debug_dyadic_cover": (_i32, [_i64, _i64, _i32, _p_i64, ctypes.POINTER(_i32), _i32]),
     "rpy_signature_gram": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "rpy_stream_signature_vjp": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp,
+                                                ctypes.c_int, _vp]),
+    "rpy_dense_ft_adjoint_right_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64,
+                                                      _i64, _i64, _i64, _vp]),
 }
 
 

@@ -307,6 +307,22 @@ int rpy_dyadic_cache_query(void* out, const void* cache, const void* increments,
                               dtype, stream);
 }
 
+int rpy_stream_signature_vjp(void* grad_increments, const void* increments,
+                             const int64_t* offsets, int64_t n_streams, int32_t width,
+                             int32_t depth, const void* signatures, const void* grad_signatures,
+                             rpy_dtype dtype, void* stream)
+{
+    return stream_signature_vjp(grad_increments, increments, offsets, n_streams, width, depth,
+                                signatures, grad_signatures, dtype, stream);
+}
+
+int rpy_dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                                   const rpy_tensor_basis* basis, int64_t batch, int64_t os,
+                                   int64_t ps, int64_t as, void* stream)
+{
+    return dense_ft_adjoint_right_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+}
+
 int rpy_signature_gram(float* gram, const float* signatures, int64_t n, int64_t k,
                        int64_t ld_signatures, int64_t ld_gram, void* stream)
 {

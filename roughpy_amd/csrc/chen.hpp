@@ -40,6 +40,14 @@ int dyadic_cache_query(void* out, const void* cache, const void* increments, con
                        rpy_dtype dtype, void* stream);
 int dyadic_cover_host(int64_t A, int64_t B, int r, int64_t* ks, int* ns, int max_pieces);
 
+// backward.hip
+int stream_signature_vjp(void* grad, const void* increments, const int64_t* offsets,
+                         int64_t n_streams, int32_t width, int32_t depth, const void* signatures,
+                         const void* grad_signatures, rpy_dtype dtype, void* stream);
+int dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                               const rpy_tensor_basis* basis, int64_t batch, int64_t os,
+                               int64_t ps, int64_t as, void* stream);
+
 // gram.hip
 int gram_bf16x3(float* G, const float* S, int64_t n, int64_t k, int64_t lds, int64_t ldg,
                 void* stream);

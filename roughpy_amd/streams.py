@@ -372,6 +372,71 @@ class LieIncrementStream:
                               dtype=dt)
         return DenseFreeTensor(sig[0], basis)
 
+    def signature_derivative(self, interval_or_perturbations, perturbation=None,
+                             resolution=None, *, depth=None, ctx=None) -> DenseFreeTensor:
+        """Directional derivative of the signature (forward mode), mirror of
+        Stream.signature_derivative (streams/src/stream.cpp:173-230) and
+        LiteContext::sig_derivative_impl (algebra/src/lite_context.h:
+        580-636): for pairs (interval_i, perturbation_i),
+            result = result (x) E_i + E_i (x) dexp_{X_i}(P_i),
+        E_i the interval's signature, X_i = log E_i, P_i = l2t(perturbation_i),
+        dexp_X(P) = sum_{d>=0} (-1)^d ad_X^d(P) / (d+1)!.
+        Forms: (interval, perturbation[, resolution]) or
+        ([(interval, perturbation), ...][, resolution])."""
+        from . import compute as rc
+        if perturbation is None or isinstance(perturbation, (int, float, np.integer,
+                                                              np.floating)):
+            pairs = list(interval_or_perturbations)
+            if perturbation is not None:
+                resolution = perturbation
+        else:
+            pairs = [(interval_or_perturbations, perturbation)]
+        res = _resolution_arg(resolution)
+        depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
+        tb = TensorBasis(self.width, depth)
+        lb = LieBasis(self.width, depth)
+        t = _device.require_gpu()
+
+        def T(x):
+            return rc.FreeTensor(x, tb)
+
+        def mul(a, b):
+            return rc.ft_mul(T(a), T(b)).data
+
+        result = t.zeros((1, tb.size()), dtype=t.float64, device="cuda")
+        for ivl, pert in pairs:
+            if isinstance(ivl, RealInterval):
+                interval = (ivl.inf, ivl.sup)
+            elif isinstance(ivl, (tuple, list)):
+                interval = (float(ivl[0]), float(ivl[1]))
+            else:
+                interval = (float(ivl.inf() if callable(ivl.inf) else ivl.inf),
+                            float(ivl.sup() if callable(ivl.sup) else ivl.sup))
+            rows = self._rows[self._select(interval, res)]
+            if rows.shape[0]:
+                offs = np.array([0, rows.shape[0]], dtype=np.int64)
+                E = signature_batch(t.from_numpy(np.ascontiguousarray(rows, dtype=np.float64)).cuda(),
+                                    t.from_numpy(offs).cuda(), self.width, depth,
+                                    dtype=np.float64)
+            else:
+                E = t.zeros((1, tb.size()), dtype=t.float64, device="cuda")
+                E[0, 0] = 1.0
+            X = rc.ft_log(T(E)).data
+            pdata = np.asarray(getattr(pert, "data", pert), dtype=np.float64).reshape(1, -1)
+            lie = np.zeros((1, lb.size()))
+            n = min(lb.size(), pdata.shape[1])
+            lie[0, :n] = pdata[0, :n]
+            P = rc.lie_to_tensor(rc.Lie(t.from_numpy(lie).cuda(), lb), tb).data
+            ad = mul(X, P) - mul(P, X)
+            series = P.clone()
+            fact = 1.0
+            for d in range(1, depth + 1):
+                fact *= d + 1
+                series = series + ad * (((-1.0) ** d) / fact)
+                ad = mul(X, ad) - mul(ad, X)
+            result = mul(result, E) + mul(E, series)
+        return DenseFreeTensor(_device.to_host(result)[0], tb)
+
     def log_signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
                       dtype=None, ctx=None) -> DenseLie:
         interval, res = self._parse(interval_or_inf, sup, resolution)

@@ -1,0 +1,172 @@
+"""GPU parity of the derivatives.
+
+* Forward mode: LieIncrementStream.signature_derivative against the
+  reference goldens (tests/streams/test_lie_increment_path.py:219-287) and
+  the oracle restatement of sig_derivative_impl.
+* Reverse mode: the hand-written reverse-Chen VJP of batched signatures,
+  checked against float64 central finite differences (step 1e-6) of the
+  oracle signature, and the float32 gradient's error against the float64
+  one stated.  Log-signature gradients (log series and t2l composed from
+  differentiable kernels) are checked the same way.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose
+
+from oracle import oracle as O
+from roughpy_amd import autograd as ag
+from roughpy_amd import streams as rs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+
+
+def test_signature_derivative_goldens(gpu):
+    g = GOLD["sig_derivative_w3_d2"]
+    for depth, expected in [(1, [0.0, 0.0, 1.0, 0.0]), (2, g["expected"])]:
+        p = rs.LieIncrementStream.from_increments(np.array([g["increment"]]),
+                                                  indices=np.array([0.0]), width=3, depth=depth)
+        d = p.signature_derivative(rs.RealInterval(0.0, 1.0), np.array(g["perturbation"]))
+        assert_allclose(d.data, expected, atol=1e-15)
+
+
+def test_signature_derivative_list_vs_oracle(gpu, rng):
+    w, d, L = 3, 3, 20
+    data = rng.normal(size=(L, w))
+    idx = np.linspace(0.0, 1.0, L)
+    p = rs.LieIncrementStream.from_increments(data, indices=idx, width=w, depth=d)
+    so = O.IncrementStream(data, w, d, indices=idx)
+    lsize = sum(O.witt_dim(w, k) for k in range(1, d + 1))
+    perts = [(rs.RealInterval(0.0, 0.3), rng.uniform(0, 5, size=w)),
+             (rs.RealInterval(0.3, 0.6), rng.uniform(0, 5, size=w)),
+             (rs.RealInterval(0.6, 1.1), rng.uniform(0, 5, size=w))]
+    got = p.signature_derivative(perts, 5).data
+    pairs = []
+    for ivl, pert in perts:
+        E = so.signature(ivl.inf, ivl.sup, 5)
+        lie = np.zeros(lsize)
+        lie[:w] = pert
+        pairs.append((E, O.lie_to_tensor(lie, w, d)))
+    assert_allclose(got, O.signature_derivative(pairs, w, d), rtol=1e-12, atol=1e-12)
+
+
+def _loss_oracle(inc, offs, G, w, d, s):
+    S = O.signature(inc[offs[s]:offs[s + 1]], w, d)
+    return float(np.dot(S, G[s]))
+
+
+@pytest.mark.parametrize("w,d", [(2, 3), (3, 4), (5, 5), (8, 4)])
+def test_signature_vjp_vs_finite_differences(gpu, rng, w, d):
+    torch = gpu
+    n = 6
+    lens = rng.integers(1, 40, size=n)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = rng.normal(size=(int(offs[-1]), w)) / np.sqrt(np.repeat(lens, lens))[:, None]
+    tsize = O.tensor_size(w, d)
+    G = rng.normal(size=(n, tsize))
+    x = torch.from_numpy(inc).cuda().requires_grad_(True)
+    S = ag.signature(x, torch.from_numpy(offs).cuda(), w, d)
+    (S * torch.from_numpy(G).cuda()).sum().backward()
+    grad = x.grad.cpu().numpy()
+    h = 1e-6
+    for _ in range(12):
+        s = int(rng.integers(0, n))
+        t = int(offs[s] + rng.integers(0, lens[s]))
+        a = int(rng.integers(0, w))
+        ip, im = inc.copy(), inc.copy()
+        ip[t, a] += h
+        im[t, a] -= h
+        fd = (_loss_oracle(ip, offs, G, w, d, s) - _loss_oracle(im, offs, G, w, d, s)) / (2 * h)
+        assert abs(grad[t, a] - fd) <= 1e-6 * max(1.0, abs(fd)), (s, t, a, grad[t, a], fd)
+
+
+def test_signature_vjp_config4_shape(gpu):
+    """Width 8, depth 4 (4681 coefficients), streams of length 256 with
+    N(0, 1/256) increments; loss <G, S> with a fixed seeded G.  Float64
+    gradient vs central differences on 64 sampled increments; float32
+    gradient's error vs float64 stated."""
+    torch = gpu
+    rng = np.random.default_rng(12345 + 4)
+    w, d, n, L = 8, 4, 4096, 256
+    inc = rng.normal(0.0, np.sqrt(1.0 / L), size=(n * L, w))
+    offs = np.arange(0, n * L + 1, L, dtype=np.int64)
+    G = rng.normal(size=(n, O.tensor_size(w, d)))
+    to = torch.from_numpy(offs).cuda()
+    x64 = torch.from_numpy(inc).cuda().requires_grad_(True)
+    (ag.signature(x64, to, w, d) * torch.from_numpy(G).cuda()).sum().backward()
+    g64 = x64.grad.cpu().numpy()
+    x32 = torch.from_numpy(inc.astype(np.float32)).cuda().requires_grad_(True)
+    (ag.signature(x32, to, w, d) * torch.from_numpy(G.astype(np.float32)).cuda()).sum().backward()
+    g32 = x32.grad.cpu().numpy()
+    err32 = np.abs(g32 - g64).max() / np.abs(g64).max()
+    print(f"config-4 shape: float32 gradient max error / max |grad| = {err32:.2e}")
+    assert err32 < 1e-3
+    h = 1e-6
+    worst = 0.0
+    for _ in range(64):
+        s = int(rng.integers(0, n))
+        t = int(offs[s] + rng.integers(0, L))
+        a = int(rng.integers(0, w))
+        ip, im = inc.copy(), inc.copy()
+        ip[t, a] += h
+        im[t, a] -= h
+        fd = (_loss_oracle(ip, offs, G, w, d, s) - _loss_oracle(im, offs, G, w, d, s)) / (2 * h)
+        worst = max(worst, abs(g64[t, a] - fd) / max(1.0, abs(fd)))
+    print(f"config-4 shape: float64 gradient vs central differences, worst {worst:.2e}")
+    assert worst < 1e-6
+
+
+def test_log_signature_gradient(gpu, rng):
+    torch = gpu
+    w, d = 3, 4
+    n = 4
+    lens = rng.integers(2, 15, size=n)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = rng.normal(size=(int(offs[-1]), w)) / 3
+    lsize = sum(O.witt_dim(w, k) for k in range(1, d + 1))
+    Gl = rng.normal(size=(n, lsize))
+    x = torch.from_numpy(inc).cuda().requires_grad_(True)
+    ls = ag.log_signature(x, torch.from_numpy(offs).cuda(), w, d)
+    for s in range(n):
+        assert_allclose(ls[s].detach().cpu().numpy(),
+                        O.log_signature(inc[offs[s]:offs[s + 1]], w, d), atol=1e-12)
+    (ls * torch.from_numpy(Gl).cuda()).sum().backward()
+    grad = x.grad.cpu().numpy()
+    h = 1e-6
+    for _ in range(10):
+        s = int(rng.integers(0, n))
+        t = int(offs[s] + rng.integers(0, lens[s]))
+        a = int(rng.integers(0, w))
+        ip, im = inc.copy(), inc.copy()
+        ip[t, a] += h
+        im[t, a] -= h
+        fp = np.dot(O.log_signature(ip[offs[s]:offs[s + 1]], w, d), Gl[s])
+        fm = np.dot(O.log_signature(im[offs[s]:offs[s + 1]], w, d), Gl[s])
+        fd = (fp - fm) / (2 * h)
+        assert abs(grad[t, a] - fd) <= 1e-6 * max(1.0, abs(fd)), (s, t, a)
+
+
+def test_ft_mul_backward_matches_adjoints(gpu, rng):
+    torch = gpu
+    from roughpy_amd.bases import TensorBasis
+    w, d = 3, 3
+    tb = TensorBasis(w, d)
+    n = tb.size()
+    a = rng.normal(size=(2, n))
+    b = rng.normal(size=(2, n))
+    g = rng.normal(size=(2, n))
+    ta = torch.from_numpy(a).cuda().requires_grad_(True)
+    tbb = torch.from_numpy(b).cuda().requires_grad_(True)
+    out = ag.ft_mul(ta, tbb, basis=tb)
+    assert_allclose(out.detach().cpu().numpy(), O.ft_mul(a, b, w, d), atol=1e-13)
+    (out * torch.from_numpy(g).cuda()).sum().backward()
+    # brute force: d<g, a b>/da_i = <g, e_i b>
+    E = np.eye(n)
+    ga = np.array([[np.dot(g[r], O.ft_mul(E[i], b[r], w, d)) for i in range(n)] for r in range(2)])
+    gb = np.array([[np.dot(g[r], O.ft_mul(a[r], E[i], w, d)) for i in range(n)] for r in range(2)])
+    assert_allclose(ta.grad.cpu().numpy(), ga, atol=1e-12)
+    assert_allclose(tbb.grad.cpu().numpy(), gb, atol=1e-12)

@@ -302,3 +302,48 @@ def test_planted_mistakes_are_caught():
     dropped_factorial[1 + 3] *= 2              # x^2 without the 1/2!
     assert not np.allclose(dropped_factorial, [1, 1, 0, 0, 0.5] + [0] * 8 + [1 / 6] + [0] * 26)
     assert np.allclose(e, [1, 1, 0, 0, 0.5] + [0] * 8 + [1 / 6] + [0] * 26)
+
+
+# --- signature derivative (forward mode) -------------------------------------
+
+def test_signature_derivative_goldens():
+    # tests/streams/test_lie_increment_path.py:219-247
+    x = np.array([0.2, 0.4, 0.6])
+    for depth, expected in [(1, [0.0, 0.0, 1.0, 0.0]),
+                            (2, GOLD["sig_derivative_w3_d2"]["expected"])]:
+        n = O.tensor_size(3, depth)
+        E = O.signature(x[None, :], 3, depth)
+        P = np.zeros(n)
+        P[2] = 1.0  # letter 2
+        got = O.signature_derivative([(E, P)], 3, depth)
+        assert_allclose(got, expected, atol=1e-15)
+
+
+def test_dexp_series_is_the_derivative_of_exp(rng):
+    # d/de exp(X + e P) = exp(X) dexp_X(P): central differences in float64
+    w, d = 3, 4
+    n = O.tensor_size(w, d)
+    X = np.zeros(n)
+    X[1:] = rng.normal(size=n - 1) * 0.3
+    X = O.tensor_to_lie(X, w, d) @ O.l2t_dense(w, d).T  # a Lie element
+    P = np.zeros(n)
+    P[1:] = rng.normal(size=n - 1)
+    P = O.tensor_to_lie(P, w, d) @ O.l2t_dense(w, d).T
+    h = 1e-6
+    fd = (O.ft_exp(X + h * P, w, d) - O.ft_exp(X - h * P, w, d)) / (2 * h)
+    an = O.ft_mul(O.ft_exp(X, w, d), O.dexp_series(X, P, w, d), w, d)
+    assert_allclose(an, fd, atol=1e-8)
+
+
+def test_signature_derivative_composition(rng):
+    # test_lie_increment_path.py:266-287: the list form equals the running
+    # product of single-interval derivatives
+    w, d = 2, 3
+    Es = [O.signature(rng.normal(size=(3, w)) / 3, w, d) for _ in range(3)]
+    Ps = [O.lie_to_tensor(rng.uniform(0, 5, size=(O.tensor_size(w, d) * 0 + 5,)), w, d)
+          for _ in range(3)]
+    whole = O.signature_derivative(list(zip(Es, Ps)), w, d)
+    exp_ = np.zeros_like(whole)
+    for E, P in zip(Es, Ps):
+        exp_ = O.ft_mul(exp_, E, w, d) + O.signature_derivative([(E, P)], w, d)
+    assert_allclose(whole, exp_, atol=1e-13)
