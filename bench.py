@@ -27,6 +27,8 @@ SEED = 12345 + 3
 # sum_{m=1..D} (D - m + 1) W^m (shared Horner partials, unit level exploited)
 FMA_PER_STEP = sum((D - m + 1) * W ** m for m in range(1, D + 1))
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X published vector peaks
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+PEAK_HBM_GBS = 8000.0
 
 
 PROFILE = os.path.join(ROOT, "profiles", "chen_signature_w5_d5_16384streams.md")
@@ -110,6 +112,183 @@ def dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs, n_queries=100000, le
     return res
 
 
+def level_rel_err(x, ref, width, depth):
+    """Max over rows and degrees of max|x - ref| / max|ref| within a degree."""
+    from oracle import oracle as O
+    db = O.tensor_degree_begin(width, depth)
+    err = 0.0
+    for k in range(depth + 1):
+        a = x[:, db[k]:db[k + 1]].double()
+        b = ref[:, db[k]:db[k + 1]].double()
+        num = (a - b).abs().amax(dim=1)
+        den = b.abs().amax(dim=1).clamp_min(1e-300)
+        err = max(err, float((num / den).max()))
+    return err
+
+
+def fma_leg(torch, steps, warmup):
+    """Batched dense free-tensor FMA C += A (x) B, width 4, depth 6 (5461
+    coefficients), batch 65536; A, B, C with unit scalar level and degree-k
+    coefficients uniform[-1, 1] / k!.  Errors are of one FMA from the same C.
+    HBM-bound: algorithmic traffic per tensor triple is read A, B, C and
+    write C."""
+    from math import factorial
+
+    from roughpy_amd import _capi, _device
+    from roughpy_amd.bases import TensorBasis
+    w, d, batch = 4, 6, 65536
+    tb = TensorBasis(w, d)
+    n = tb.size()
+    db = tb.degree_begin
+    cs = tb.c_struct()
+    scale = torch.empty(n, dtype=torch.float32, device="cuda")
+    for k in range(d + 1):
+        scale[db[k]:db[k + 1]] = 1.0 / factorial(k)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 20)
+    A32 = (torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale
+    B32 = (torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale
+    C0 = (torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale
+    A32[:, 0] = 1.0
+    B32[:, 0] = 1.0
+    C0[:, 0] = 1.0
+    L = _capi.lib()
+    Bd = _capi.DegreeBoundsC
+    fma_per = sum((m + 1) * w ** m for m in range(d + 1))
+    res = {"workload": f"C += A (x) B, width {w}, depth {d} ({n} coefficients), batch {batch}",
+           "fma_per_product": fma_per}
+    outs = {}
+    for name, code, A, B in (("f64", _capi.RPY_F64, A32.double(), B32.double()),
+                             ("f32", _capi.RPY_F32, A32, B32),
+                             ("f32_compensated", _capi.RPY_F32_COMPENSATED, A32, B32)):
+        C = C0.to(A.dtype, copy=True)
+
+        def fn():
+            _capi.check(L.rpy_dense_ft_fma(C.data_ptr(), A.data_ptr(), B.data_ptr(), code,
+                                           cs, Bd(0, -1), Bd(0, -1), Bd(0, -1),
+                                           batch, n, n, n, _device.stream_handle()))
+        try:
+            fn()
+        except NotImplementedError:
+            continue
+        torch.cuda.synchronize()
+        outs[name] = C.clone()
+        ms = time_kernel(fn, steps, warmup, torch)
+        es = A.element_size()
+        gbs = 4.0 * n * es * batch / (ms * 1e-3) / 1e9
+        res[name] = {"ms_per_step": ms, "products_per_s": batch / (ms * 1e-3),
+                     "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+                     "tflops": 2.0 * fma_per * batch / (ms * 1e-3) / 1e12}
+        del C
+    for name in ("f32", "f32_compensated"):
+        if name in outs:
+            res[name]["rel_err_vs_f64"] = level_rel_err(outs[name], outs["f64"], w, d)
+    del outs, A32, B32, C0
+    torch.cuda.empty_cache()
+    return res
+
+
+def backward_leg(torch, steps, warmup):
+    """Signature forward + reverse-Chen backward, width 8, depth 4 (4681
+    coefficients), 4096 streams of length 256, N(0, 1/256) increments, loss
+    <G, S> with a seeded random G.  Algorithmic FLOPs of the backward: four
+    Chen-step equivalents per increment (prefix recovery by exp(-x), partial
+    recomputation, and the two adjoint FMAs of every forward FMA)."""
+    from roughpy_amd import _capi, _device
+    w, d, n, L_ = 8, 4, 4096, 256
+    rng = np.random.default_rng(SEED + 30)
+    inc32 = torch.from_numpy(rng.normal(0.0, np.sqrt(1.0 / L_), size=(n * L_, w))
+                             .astype(np.float32)).cuda()
+    offs = torch.arange(0, n * L_ + 1, L_, dtype=torch.int64, device="cuda")
+    tsize = int(_capi.lib().rpy_tensor_size(w, d))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 31)
+    G32 = torch.randn((n, tsize), generator=gen, device="cuda")
+    fma_step = sum((d - m + 1) * w ** m for m in range(1, d + 1))
+    lib = _capi.lib()
+    res = {"workload": f"signature + backward, width {w}, depth {d} ({tsize} coefficients), "
+                       f"{n} streams of length {L_}", "fma_per_step": fma_step}
+    grads = {}
+    for name, code, inc, G in (("f64", _capi.RPY_F64, inc32.double(), G32.double()),
+                               ("f32", _capi.RPY_F32, inc32, G32)):
+        S = torch.empty((n, tsize), dtype=inc.dtype, device="cuda")
+        grad = torch.zeros_like(inc)
+
+        def fwd():
+            _capi.check(lib.rpy_stream_signature_batch(S.data_ptr(), inc.data_ptr(),
+                                                       offs.data_ptr(), n, w, d, code,
+                                                       _device.stream_handle()))
+
+        def bwd():
+            _capi.check(lib.rpy_stream_signature_vjp(grad.data_ptr(), inc.data_ptr(),
+                                                     offs.data_ptr(), n, w, d, S.data_ptr(),
+                                                     G.data_ptr(), code, _device.stream_handle()))
+        fwd_ms = time_kernel(fwd, steps, warmup, torch)
+        bwd_ms = time_kernel(bwd, max(1, steps // 2), 1, torch)
+        grads[name] = grad.clone()
+        peak = PEAK_TFLOPS[name]
+        f_tf = 2.0 * fma_step * n * L_ / (fwd_ms * 1e-3) / 1e12
+        b_tf = 2.0 * 4 * fma_step * n * L_ / (bwd_ms * 1e-3) / 1e12
+        res[name] = {"forward_ms": fwd_ms, "backward_ms": bwd_ms,
+                     "forward_tflops": f_tf, "forward_frac": f_tf / peak,
+                     "backward_tflops": b_tf, "backward_frac": b_tf / peak,
+                     "gradients_per_s": n / ((fwd_ms + bwd_ms) * 1e-3)}
+        del S, grad
+    g64, g32 = grads["f64"], grads["f32"].double()
+    res["f32"]["grad_rel_err_vs_f64"] = float((g32 - g64).abs().max() / g64.abs().max())
+    del grads, inc32, G32
+    torch.cuda.empty_cache()
+    return res
+
+
+def gram_leg(torch, steps, warmup):
+    """Signature kernel by signature dot: 2048 streams, width 6, length 512,
+    N(0, 1/512) increments, depth-5 float32 signatures (9331 coefficients),
+    then the 2048 x 2048 Gram matrix on MFMA (bf16 hi/lo split, three passes,
+    upper-triangle tiles).  Error vs a float64 GEMM of the same signatures."""
+    from roughpy_amd import _capi, _device
+    w, d, n, L_ = 6, 5, 2048, 512
+    rng = np.random.default_rng(SEED + 40)
+    inc = torch.from_numpy(rng.normal(0.0, np.sqrt(1.0 / L_), size=(n * L_, w))
+                           .astype(np.float32)).cuda()
+    offs = torch.arange(0, n * L_ + 1, L_, dtype=torch.int64, device="cuda")
+    lib = _capi.lib()
+    k = int(lib.rpy_tensor_size(w, d))
+    S = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    Gm = torch.empty((n, n), dtype=torch.float32, device="cuda")
+
+    def sig():
+        _capi.check(lib.rpy_stream_signature_batch(S.data_ptr(), inc.data_ptr(), offs.data_ptr(),
+                                                   n, w, d, _capi.RPY_F32,
+                                                   _device.stream_handle()))
+
+    def gram():
+        _capi.check(lib.rpy_signature_gram(Gm.data_ptr(), S.data_ptr(), n, k, k, n,
+                                           _device.stream_handle()))
+    sig_ms = time_kernel(sig, steps, warmup, torch)
+    gram_ms = time_kernel(gram, steps, warmup, torch)
+    S64 = S.double()
+    ref = S64 @ S64.T
+    dg = torch.sqrt(torch.outer(torch.diagonal(ref), torch.diagonal(ref)))
+    rel = float(((Gm.double() - ref).abs() / dg).max())
+    fma_step = sum((d - m + 1) * w ** m for m in range(1, d + 1))
+    sig_tf = 2.0 * fma_step * n * L_ / (sig_ms * 1e-3) / 1e12
+    # algorithmic work is one n x n x k GEMM; the three bf16 passes and the
+    # upper-triangle tiling are the kernel's own business
+    gemm_flops = 2.0 * n * n * k
+    gram_tf = gemm_flops / (gram_ms * 1e-3) / 1e12
+    res = {"workload": f"{n} signatures, width {w}, depth {d} ({k} coefficients), length {L_}, "
+                       f"then the {n} x {n} Gram matrix",
+           "signature_ms": sig_ms, "signature_tflops": sig_tf,
+           "signature_frac": sig_tf / PEAK_TFLOPS["f32"],
+           "gram_ms": gram_ms, "gram_bf16_tflops": gram_tf,
+           "gram_frac": gram_tf / PEAK_BF16_TFLOPS,
+           "gram_max_rel_err_vs_f64_gemm": rel}
+    del S, Gm, S64, ref, inc
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(inc, offs, target_seconds=10.0):
     """Reference CPU path on a bounded sample: the reference's own
     roughpy_compute ft_fmexp per increment (oracle/_ref/libref.so, built from
@@ -150,6 +329,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="dyadic,fma,backward,gram",
+                    help="comma-separated secondary workloads to run after the headline "
+                         "(dyadic, fma, backward, gram; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -201,18 +383,10 @@ def main():
     ms, out = results[args.dtype]
     flops = 2.0 * FMA_PER_STEP * total_incr
     achieved = flops / (ms * 1e-3) / 1e12
-    s32 = results["f32"][1].double()
+    s32 = results["f32"][1]
     s64 = results["f64"][1]
     # per-level normwise relative error of the float32 kernel vs the float64 kernel
-    from oracle import oracle as O
-    db = O.tensor_degree_begin(W, D)
-    err = 0.0
-    for k in range(D + 1):
-        a = s32[:, db[k]:db[k + 1]]
-        b = s64[:, db[k]:db[k + 1]]
-        num = (a - b).abs().amax(dim=1)
-        den = b.abs().amax(dim=1).clamp_min(1e-300)
-        err = max(err, float((num / den).max()))
+    err = level_rel_err(s32, s64, W, D)
 
     line = {
         "metric": "signatures/sec, w=5 N=5 batched; % VALU/HBM roofline; fp32 vs fp64 rel-err",
@@ -249,7 +423,17 @@ def main():
                        "tflops": flops / (results["f32"][0] * 1e-3) / 1e12},
         "rel_err_f32_vs_f64": err,
     }
-    line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
+    legs = [x for x in args.legs.split(",") if x]
+    if "dyadic" in legs:
+        line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
+    del dinc64, results, s32, s64
+    torch.cuda.empty_cache()
+    if "fma" in legs:
+        line["fma_w4_d6"] = fma_leg(torch, max(3, args.steps // 2), 1)
+    if "backward" in legs:
+        line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
+    if "gram" in legs:
+        line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(inc, offs)
     print(json.dumps(line), flush=True)

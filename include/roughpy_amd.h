@@ -36,7 +36,8 @@ typedef enum rpy_status {
 } rpy_status;
 
 /* Arithmetic type of a call.  RPY_F32_COMPENSATED reads and writes float
- * data but carries each signature coefficient as a compensated pair. */
+ * data and carries sums beyond float precision; each entry point that
+ * accepts it states how. */
 typedef enum rpy_dtype {
     RPY_F32 = 0,
     RPY_F64 = 1,
@@ -130,7 +131,9 @@ int rpy_t2l_matrix(const rpy_lie_basis *basis, int64_t *nnz, int64_t *offsets,
 
 /* out += lhs (x) rhs, truncated; replaces dense_ft_fma
  * (roughpy/compute/_src/dense_basic.cpp:85-151, kernel
- * roughpy_compute/dense/basic/free_tensor_fma.hpp:13-53). */
+ * roughpy_compute/dense/basic/free_tensor_fma.hpp:13-53).  dtype RPY_F32,
+ * RPY_F64, or RPY_F32_COMPENSATED (float data, each output coefficient
+ * accumulated in double and rounded to float once). */
 int rpy_dense_ft_fma(void *out, const void *lhs, const void *rhs,
                      rpy_dtype dtype, const rpy_tensor_basis *basis,
                      rpy_degree_bounds out_bounds, rpy_degree_bounds lhs_bounds,

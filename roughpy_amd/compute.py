@@ -146,10 +146,15 @@ def _padded(a, basis_size: int, dtype, name: str):
 # ---------------------------------------------------------------------------
 
 def ft_fma(a: FreeTensor, b: FreeTensor, c: FreeTensor, **kwargs):
-    """Free tensor fused multiply-add: a = b * c + a (in place on a.data)."""
+    """Free tensor fused multiply-add: a = b * c + a (in place on a.data).
+
+    ``compensated=True`` with float32 data accumulates every output
+    coefficient in double and rounds it to float once."""
     _check_basis_compat(a.basis, b.basis, c.basis)
     dt = _device.np_dtype(a.data)
     code = _dtype_code(dt)
+    if kwargs.get("compensated") and dt == np.float32:
+        code = _capi.RPY_F32_COMPENSATED
     _check_leading(a.data, b.data, c.data)
     basis = a.basis
     size = basis.size()
@@ -170,7 +175,7 @@ def ft_mul(a: FreeTensor, b: FreeTensor, **kwargs) -> FreeTensor:
     dt = _device.np_dtype(a.data)
     _dtype_code(dt)
     res = _new_like(_shape(a.data), dt)
-    ft_fma(FreeTensor(res, a.basis), a, b)
+    ft_fma(FreeTensor(res, a.basis), a, b, **kwargs)
     return DenseFreeTensor(_wrap_result(res, a.data), a.basis)
 
 

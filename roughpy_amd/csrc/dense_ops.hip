@@ -46,7 +46,10 @@ int grid_for(int64_t batch)
 // ---------------------------------------------------------------------------
 // out += lhs (x) rhs
 // ---------------------------------------------------------------------------
-template <typename T, bool Stage>
+// Acc is the accumulator type: T itself, or double for float data in the
+// compensated mode (products of two floats are exact in double, so each
+// output coefficient is rounded to float once).
+template <typename T, typename Acc, bool Stage>
 __global__ __launch_bounds__(kBlock) void ft_fma_kernel(
         T* __restrict__ out, const T* __restrict__ lhs, const T* __restrict__ rhs,
         DevBasis b, Pow pw, Bounds ob, Bounds lb, Bounds rb, int64_t batch, int64_t os,
@@ -78,19 +81,20 @@ __global__ __launch_bounds__(kBlock) void ft_fma_kernel(
             if (lmax < lmin) continue;
             const uint32_t n = uint32_t(b.level_size(od));
             for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
-                T acc = o[b.db[od] + idx];
+                Acc acc = Acc(o[b.db[od] + idx]);
                 for (int ld = lmax; ld >= lmin; --ld) {
                     const int rd = od - ld;
                     const uint32_t i = fdiv(idx, pw.pw[rd]);
                     const uint32_t j = idx - i * pw.pw[rd].d;
                     const T lv = (ld < top) ? L[b.db[ld] + i] : l[b.db[ld] + i];
                     const T rv = (rd < top) ? R[b.db[rd] + j] : r[b.db[rd] + j];
-                    acc += lv * rv;
+                    acc += Acc(lv) * Acc(rv);
                 }
-                o[b.db[od] + idx] = acc;
+                o[b.db[od] + idx] = T(acc);
             }
         }
-        if (threadIdx.x == 0 && ob.lo == 0 && lb.lo == 0 && rb.lo == 0) o[0] += l[0] * r[0];
+        if (threadIdx.x == 0 && ob.lo == 0 && lb.lo == 0 && rb.lo == 0)
+            o[0] = T(Acc(o[0]) + Acc(l[0]) * Acc(r[0]));
         if (Stage) __syncthreads();
     }
 }
@@ -338,7 +342,7 @@ int check_batch(int64_t batch, std::initializer_list<int64_t> strides, int64_t n
     return RPY_OK;
 }
 
-template <typename T>
+template <typename T, typename Acc = T>
 int launch_fma(void* out, const void* lhs, const void* rhs, const DevBasis& b, Bounds ob,
                Bounds lb, Bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
                hipStream_t st)
@@ -348,11 +352,11 @@ int launch_fma(void* out, const void* lhs, const void* rhs, const DevBasis& b, B
     const size_t lds = size_t(2) * size_t(b.db[top]) * sizeof(T);
     dim3 grid(grid_for(batch)), block(kBlock);
     if (lds <= kLdsBudget) {
-        hipLaunchKernelGGL((ft_fma_kernel<T, true>), grid, block, lds, st, (T*)out,
+        hipLaunchKernelGGL((ft_fma_kernel<T, Acc, true>), grid, block, lds, st, (T*)out,
                            (const T*)lhs, (const T*)rhs, b, pw, ob, lb, rb, batch, os, ls, rs,
                            top);
     } else {
-        hipLaunchKernelGGL((ft_fma_kernel<T, false>), grid, block, 0, st, (T*)out,
+        hipLaunchKernelGGL((ft_fma_kernel<T, Acc, false>), grid, block, 0, st, (T*)out,
                            (const T*)lhs, (const T*)rhs, b, pw, ob, lb, rb, batch, os, ls, rs,
                            0);
     }
@@ -426,7 +430,8 @@ int dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
 {
     DevBasis b;
     int rc;
-    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    if ((rc = make_basis(basis, b))) return rc;
+    if (dtype != RPY_F32_COMPENSATED && (rc = plain_dtype(dtype))) return rc;
     Bounds ob, lb, rb;
     if ((rc = check_bounds(b, obi, ob, "out")) || (rc = check_bounds(b, lbi, lb, "lhs")) ||
         (rc = check_bounds(b, rbi, rb, "rhs")))
@@ -434,11 +439,17 @@ int dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
     if ((rc = check_batch(batch, {os, ls, rs}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !lhs || !rhs) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
-    return dtype == RPY_F64
-                   ? launch_fma<double>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
-                                        as_stream(stream))
-                   : launch_fma<float>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
-                                       as_stream(stream));
+    switch (dtype) {
+        case RPY_F64:
+            return launch_fma<double>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
+                                      as_stream(stream));
+        case RPY_F32_COMPENSATED:
+            return launch_fma<float, double>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
+                                             as_stream(stream));
+        default:
+            return launch_fma<float>(out, lhs, rhs, b, ob, lb, rb, batch, os, ls, rs,
+                                     as_stream(stream));
+    }
 }
 
 int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,

@@ -81,6 +81,8 @@ __global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ 
     const int bj = bi + int(t);
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
+    // the lower sub-tile of a diagonal tile is the mirror of the upper one
+    if (bi == bj && wave == 2) return;
     const int64_t row0 = int64_t(bi) * kTile + 32 * (wave / 2);
     const int64_t col0 = int64_t(bj) * kTile + 32 * (wave % 2);
     const int64_t ra = row0 + lane % 32;  // A operand row of this lane
@@ -106,9 +108,10 @@ __global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int64_t i = row0 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        if (i < n && j < n) {
+        // on diagonal tiles only i <= j is kept, so G is exactly symmetric
+        if (i < n && j < n && (bi != bj || i <= j)) {
             G[i * ldg + j] = acc[r];
-            if (bi != bj) G[j * ldg + i] = acc[r];
+            G[j * ldg + i] = acc[r];
         }
     }
 }

@@ -182,3 +182,25 @@ def test_antipode_and_group_inverse(gpu, rng):
     ident = np.zeros_like(x)
     ident[:, 0] = 1
     assert_allclose(prod.data, ident, atol=1e-14)  # tests/compute/test_antipode.py:10-37
+
+
+@pytest.mark.parametrize("w,d", [(2, 5), (4, 6), (5, 4)])
+def test_ft_fma_compensated_is_rounded_once(gpu, rng, w, d):
+    """Compensated float32 FMA: each coefficient is the float64 result on the
+    same float32 inputs rounded once, so it is within one float32 ulp of the
+    float64 oracle (and never worse than plain float32)."""
+    batch = (17,)
+    tb = rc.TensorBasis(w, d)
+    A = _rand(rng, w, d, batch, np.float32, unit=1.0)
+    B = _rand(rng, w, d, batch, np.float32, unit=1.0)
+    C = _rand(rng, w, d, batch, np.float32, unit=1.0)
+    ref = C.astype(np.float64)
+    O.ft_fma(ref, A.astype(np.float64), B.astype(np.float64), w, d)
+    out = rc.FreeTensor(C.copy(), tb)
+    rc.ft_fma(out, rc.FreeTensor(A, tb), rc.FreeTensor(B, tb), compensated=True)
+    got = out.data.astype(np.float64)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= ulp + 1e-30), np.abs(got - ref).max()
+    plain = rc.FreeTensor(C.copy(), tb)
+    rc.ft_fma(plain, rc.FreeTensor(A, tb), rc.FreeTensor(B, tb))
+    assert np.abs(got - ref).max() <= np.abs(plain.data.astype(np.float64) - ref).max()

@@ -84,7 +84,7 @@ def test_signature_vjp_vs_finite_differences(gpu, rng, w, d):
         assert abs(grad[t, a] - fd) <= 1e-6 * max(1.0, abs(fd)), (s, t, a, grad[t, a], fd)
 
 
-def test_signature_vjp_config4_shape(gpu):
+def test_signature_vjp_w8_d4_4096_streams(gpu):
     """Width 8, depth 4 (4681 coefficients), streams of length 256 with
     N(0, 1/256) increments; loss <G, S> with a fixed seeded G.  Float64
     gradient vs central differences on 64 sampled increments; float32
@@ -103,7 +103,7 @@ def test_signature_vjp_config4_shape(gpu):
     (ag.signature(x32, to, w, d) * torch.from_numpy(G.astype(np.float32)).cuda()).sum().backward()
     g32 = x32.grad.cpu().numpy()
     err32 = np.abs(g32 - g64).max() / np.abs(g64).max()
-    print(f"config-4 shape: float32 gradient max error / max |grad| = {err32:.2e}")
+    print(f"w8 d4 4096x256: float32 gradient max error / max |grad| = {err32:.2e}")
     assert err32 < 1e-3
     h = 1e-6
     worst = 0.0
@@ -116,7 +116,7 @@ def test_signature_vjp_config4_shape(gpu):
         im[t, a] -= h
         fd = (_loss_oracle(ip, offs, G, w, d, s) - _loss_oracle(im, offs, G, w, d, s)) / (2 * h)
         worst = max(worst, abs(g64[t, a] - fd) / max(1.0, abs(fd)))
-    print(f"config-4 shape: float64 gradient vs central differences, worst {worst:.2e}")
+    print(f"w8 d4 4096x256: float64 gradient vs central differences, worst {worst:.2e}")
     assert worst < 1e-6
 
 

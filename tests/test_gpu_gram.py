@@ -27,7 +27,7 @@ def test_gram_random(gpu, rng, n, k):
     np.testing.assert_array_equal(G, G.T)
 
 
-def test_gram_of_signatures_config5_shape(gpu):
+def test_gram_of_signatures_w6_d5(gpu):
     """Signatures of 512 streams, width 6, depth 5 (9331 coefficients),
     length 512, then their Gram matrix."""
     rng = np.random.default_rng(12345 + 5)

@@ -86,7 +86,7 @@ def test_log_signature_batch_vs_oracle(gpu, rng):
             assert_allclose(got[s], ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
-def test_config1_single_stream(gpu):
+def test_brownian_w3_d4_1000_increments(gpu):
     """One stream, width 3, 1000 increments ~ N(0, 1/1000), depth 4: signature
     (121) and log-signature (32) through LieIncrementStream."""
     rng = np.random.default_rng(12345 + 1)
