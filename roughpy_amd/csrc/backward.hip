@@ -15,8 +15,14 @@
 //   adjP_M = H_M,   adjP_{K-1}(w') = sum_a adjP_K(w'a) x_a / (M-K+1),
 //   dl/dx_a += sum_{w'} adjP_K(w'a) P_{K-1}(w') / (M-K+1),
 //   H_{t-1,K} = sum_{M >= K} adjP^{(M)}_K   (= R*_{exp(x_t)} H_t).
-// One workgroup owns one stream; its four working tensors live in a
-// per-workgroup global scratch slot (L2-resident).
+// Two kernels:
+// * sig_vjp_wave_kernel: one wave64 per stream, the 64 lanes splitting the
+//   words by prefix (chen_bwd.hpp), A and H in registers, cross-lane sums by
+//   shuffles; for the shapes whose prefix split has exactly 64 lanes and
+//   whose state fits the register file (width 8 depth <= 4, width 4 depth
+//   <= 6, width 2 depth 6..10 in float32).
+// * sig_vjp_kernel: any shape; one workgroup per stream with its four
+//   working tensors in LDS when they fit, else in a global scratch slot.
 //
 // Also the adjoint of right multiplication, R*_B(g)(u) = sum_v g(uv) B(v),
 // the right-hand counterpart of free_tensor_adjoint_left_mul.hpp:10-77.
@@ -24,8 +30,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <type_traits>
+#include <vector>
 
 #include "chen.hpp"
+#include "chen_bwd.hpp"
 #include "kernel_util.hpp"
 #include "rpy_common.hpp"
 
@@ -79,16 +89,17 @@ RPY_FI T forward_partial(const T* A, const T* x, const T* inv, const DevBasis& b
     return p;
 }
 
-template <typename T>
+template <typename T, bool InLds>
 __global__ __launch_bounds__(kBlock) void sig_vjp_kernel(
         T* __restrict__ grad, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
         int64_t n_streams, const T* __restrict__ sig, const T* __restrict__ gsig,
         T* __restrict__ scratch, DevBasis b, Pow pw)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int W = b.width;
     const int D = b.depth;
     const int64_t tsize = b.size();
-    T* A = scratch + int64_t(blockIdx.x) * 4 * tsize;
+    T* A = InLds ? reinterpret_cast<T*>(smem) : scratch + int64_t(blockIdx.x) * 4 * tsize;
     T* Hc = A + tsize;
     T* Hn = Hc + tsize;
     T* Q = Hn + tsize;
@@ -194,7 +205,208 @@ __global__ __launch_bounds__(kBlock) void adj_rmul_kernel(T* __restrict__ out,
     }
 }
 
+template <int W, int P>
+RPY_HD inline void prefix_digits(int q, int (&u)[P])
+{
+    for (int j = P - 1; j >= 0; --j) {
+        u[j] = q % W;
+        q /= W;
+    }
+}
+
+// Loads lane q's share of a dense tensor (path words and subtree).
+template <typename T, int W, int D, int P>
+RPY_HD inline __attribute__((always_inline)) void load_share(T* path, T* sub, const T* t,
+                                                             const DevBasis& b, int q)
+{
+    using Sh = chen::Shape<W, D, P>;
+    chen::static_for<P>([&](auto k0) {
+        constexpr int k = int(decltype(k0)::value);
+        path[k] = t[b.db[k] + q / int(ipow(W, P - k))];
+    });
+    chen::static_for<D - P + 1>([&](auto j0) {
+        constexpr int k = P + int(decltype(j0)::value);
+        constexpr int nk = int(ipow(W, k - P));
+        chen::static_for<nk>([&](auto v0) {
+            constexpr int v = int(decltype(v0)::value);
+            sub[Sh::off(k) + v] = t[b.db[k] + int64_t(q) * nk + v];
+        });
+    });
+}
+
+template <typename T, int W, int D, int P>
+__global__ __launch_bounds__(64) void sig_vjp_wave_kernel(
+        T* __restrict__ grad, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
+        const T* __restrict__ sig, const T* __restrict__ gsig, DevBasis b)
+{
+    using Sh = chen::Shape<W, D, P>;
+    static_assert(Sh::NP == 64, "one stream per wave64");
+    constexpr int LW = (W == 2) ? 1 : (W == 4) ? 2 : 3;
+    static_assert((1 << LW) == W, "width must be 2, 4 or 8");
+    const int64_t s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t tsize = b.size();
+    int u[P];
+    prefix_digits<W, P>(lane, u);
+    chen::BwdLane<T, W, D, P> st;
+    st.set_prefix(u);
+    load_share<T, W, D, P>(st.A.path, st.A.sub, sig + s * tsize, b, lane);
+    load_share<T, W, D, P>(st.hpath, st.hsub, gsig + s * tsize, b, lane);
+    const int64_t r0 = offsets[s], r1 = offsets[s + 1];
+    for (int64_t row = r1 - 1; row >= r0; --row) {
+        const T* xr = incr + row * W;
+        T tab[D * W];
+        chen::build_table<T, W, D>(tab, xr);
+        T pu[P];
+        for (int j = 0; j < P; ++j) pu[j] = xr[u[j]];
+        st.A.step(chen::RegRow<T, W, D, P, true>{tab, pu});
+        st.contributions(chen::RegRow<T, W, D, P, false>{tab, pu});
+        // path cotangents: sum the shares over the W^(P-k) lanes below each
+        // path word (aligned lane groups)
+        chen::static_for<P - 1>([&](auto i) {
+            constexpr int k = int(decltype(i)::value) + 1;
+            constexpr int grp = int(ipow(W, P - k));
+            T v = st.dpath[k];
+            for (int off = 1; off < grp; off <<= 1) v += __shfl_xor(v, off);
+            st.hpath[k] += v;
+        });
+        // gradient: W values summed over the wave by a reduce-scatter; lane
+        // ends with the total of letter lane >> (6 - LW)
+        T v[W];
+        chen::static_for<W>([&](auto a) { v[decltype(a)::value] = st.g[decltype(a)::value]; });
+        chen::static_for<P>([&](auto j) {
+            chen::static_for<W>([&](auto a) {
+                constexpr int ai = int(decltype(a)::value);
+                if (u[decltype(j)::value] == ai) v[ai] += st.gp[decltype(j)::value];
+            });
+        });
+        chen::static_for<LW>([&](auto i0) {
+            constexpr int i = int(decltype(i0)::value);
+            constexpr int half = (W >> i) / 2;
+            constexpr int sft = 32 >> i;
+            const bool upper = (lane & sft) != 0;
+            chen::static_for<half>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const T send = upper ? v[k] : v[k + half];
+                const T keep = upper ? v[k + half] : v[k];
+                v[k] = keep + __shfl_xor(send, sft);
+            });
+        });
+        T tot = v[0];
+        for (int sft = 32 >> LW; sft >= 1; sft >>= 1) tot += __shfl_xor(tot, sft);
+        if ((lane & ((64 >> LW) - 1)) == 0) grad[row * W + (lane >> (6 - LW))] = tot;
+    }
+}
+
+// Host run of the wave kernel's lane algebra for one float64 stream (the
+// cross-lane sums in plain loops).
+template <int W, int D, int P>
+void vjp_emulate(double* grad, const double* incr, int64_t len, const double* sig,
+                 const double* gsig, const DevBasis& b)
+{
+    using Sh = chen::Shape<W, D, P>;
+    std::vector<chen::BwdLane<double, W, D, P>> lanes(Sh::NP);
+    std::vector<std::array<int, P>> digits(Sh::NP);
+    for (int q = 0; q < Sh::NP; ++q) {
+        int u[P];
+        prefix_digits<W, P>(q, u);
+        for (int j = 0; j < P; ++j) digits[q][j] = u[j];
+        lanes[q].set_prefix(u);
+        load_share<double, W, D, P>(lanes[q].A.path, lanes[q].A.sub, sig, b, q);
+        load_share<double, W, D, P>(lanes[q].hpath, lanes[q].hsub, gsig, b, q);
+    }
+    for (int64_t row = len - 1; row >= 0; --row) {
+        const double* xr = incr + row * W;
+        double tab[D * W];
+        chen::build_table<double, W, D>(tab, xr);
+        for (int q = 0; q < Sh::NP; ++q) {
+            int u[P];
+            double pu[P];
+            for (int j = 0; j < P; ++j) {
+                u[j] = digits[q][j];
+                pu[j] = xr[u[j]];
+            }
+            lanes[q].A.step(chen::RegRow<double, W, D, P, true>{tab, pu});
+            lanes[q].contributions(chen::RegRow<double, W, D, P, false>{tab, pu});
+        }
+        for (int k = 1; k < P; ++k) {
+            const int grp = int(ipow(W, P - k));
+            for (int g0 = 0; g0 < Sh::NP; g0 += grp) {
+                double sum = 0.0;
+                for (int q = g0; q < g0 + grp; ++q) sum += lanes[q].dpath[k];
+                for (int q = g0; q < g0 + grp; ++q) lanes[q].hpath[k] += sum;
+            }
+        }
+        for (int a = 0; a < W; ++a) {
+            double sum = 0.0;
+            for (int q = 0; q < Sh::NP; ++q) {
+                int u[P];
+                for (int j = 0; j < P; ++j) u[j] = digits[q][j];
+                sum += lanes[q].grad_share(a, u);
+            }
+            grad[row * W + a] = sum;
+        }
+    }
+}
+
+template <typename T, int W, int D, int P>
+void launch_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n,
+                 const void* sig, const void* gsig, const DevBasis& b, hipStream_t st)
+{
+    hipLaunchKernelGGL((sig_vjp_wave_kernel<T, W, D, P>), dim3(uint32_t(n)), dim3(64), 0, st,
+                       (T*)grad, (const T*)incr, offsets, (const T*)sig, (const T*)gsig, b);
+}
+
+// Shapes with a 64-lane prefix split whose lane state fits in registers.
+template <typename T>
+bool try_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n, const void* sig,
+              const void* gsig, const DevBasis& b, hipStream_t st)
+{
+    const int w = b.width, d = b.depth;
+#define RPY_WAVE(W, D, P)                                                  \
+    if (w == W && d == D) {                                                \
+        launch_wave<T, W, D, P>(grad, incr, offsets, n, sig, gsig, b, st); \
+        return true;                                                       \
+    }
+    RPY_WAVE(8, 2, 2) RPY_WAVE(8, 3, 2)
+    RPY_WAVE(4, 3, 3) RPY_WAVE(4, 4, 3) RPY_WAVE(4, 5, 3)
+    RPY_WAVE(2, 6, 6) RPY_WAVE(2, 7, 6) RPY_WAVE(2, 8, 6) RPY_WAVE(2, 9, 6)
+    if constexpr (std::is_same<T, float>::value) {
+        RPY_WAVE(8, 4, 2) RPY_WAVE(4, 6, 3) RPY_WAVE(2, 10, 6)
+    }
+#undef RPY_WAVE
+    return false;
+}
+
+size_t lds_limit(hipStream_t)
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 64 * 1024;
+    return size_t(v);
+}
+
 }  // namespace
+
+int vjp_host_emulation(double* grad, const double* incr, int64_t len, int32_t width,
+                       int32_t depth, const double* sig, const double* gsig)
+{
+    rpy_tensor_basis tb{width, depth, nullptr};
+    DevBasis b;
+    int rc = make_basis(&tb, b);
+    if (rc) return rc;
+#define RPY_EMU(W, D, P)                                           \
+    if (width == W && depth == D) {                                \
+        vjp_emulate<W, D, P>(grad, incr, len, sig, gsig, b);       \
+        return RPY_OK;                                             \
+    }
+    RPY_EMU(8, 2, 2) RPY_EMU(8, 3, 2) RPY_EMU(8, 4, 2)
+    RPY_EMU(4, 3, 3) RPY_EMU(4, 4, 3) RPY_EMU(4, 5, 3) RPY_EMU(4, 6, 3)
+    RPY_EMU(2, 6, 6) RPY_EMU(2, 7, 6)
+#undef RPY_EMU
+    return fail(RPY_ERR_UNSUPPORTED, "no host emulation of the wave backward for this shape");
+}
 
 int stream_signature_vjp(void* grad, const void* increments, const int64_t* offsets,
                          int64_t n_streams, int32_t width, int32_t depth, const void* signatures,
@@ -211,26 +423,70 @@ int stream_signature_vjp(void* grad, const void* increments, const int64_t* offs
     if (n_streams == 0) return RPY_OK;
     if (!grad || !increments || !offsets || !signatures || !grad_signatures)
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    if (n_streams > int64_t(0x7fffffff))
+        return fail(RPY_ERR_SHAPE, "too many streams for one launch");
     const Pow pw = make_pow(width, depth);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const bool wave = dtype == RPY_F64
+                              ? try_wave<double>(grad, increments, offsets, n_streams, signatures,
+                                                 grad_signatures, b, st)
+                              : try_wave<float>(grad, increments, offsets, n_streams, signatures,
+                                                grad_signatures, b, st);
+    if (wave) {
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
     const int64_t grid = std::min<int64_t>(n_streams, 2048);
     const size_t es = dtype == RPY_F64 ? 8 : 4;
+    const size_t lds = 4 * size_t(b.size()) * es;
     void* scratch = nullptr;
-    RPY_HIP_CHECK(hipMallocAsync(&scratch, size_t(grid) * 4 * size_t(b.size()) * es, st));
-    if (dtype == RPY_F64)
-        hipLaunchKernelGGL(sig_vjp_kernel<double>, dim3(uint32_t(grid)), dim3(kBlock), 0, st,
-                           (double*)grad, (const double*)increments, offsets, n_streams,
-                           (const double*)signatures, (const double*)grad_signatures,
-                           (double*)scratch, b, pw);
-    else
-        hipLaunchKernelGGL(sig_vjp_kernel<float>, dim3(uint32_t(grid)), dim3(kBlock), 0, st,
-                           (float*)grad, (const float*)increments, offsets, n_streams,
-                           (const float*)signatures, (const float*)grad_signatures,
-                           (float*)scratch, b, pw);
+    // the kernel's static LDS (increment, reciprocals, reduction) is < 4 KiB
+    const bool in_lds = lds + 4096 <= lds_limit(st);
+    if (!in_lds)
+        RPY_HIP_CHECK(hipMallocAsync(&scratch, size_t(grid) * 4 * size_t(b.size()) * es, st));
+    const dim3 g = dim3(uint32_t(grid)), blk = dim3(kBlock);
+    const size_t sh = in_lds ? lds : 0;
+    if (in_lds) {
+        hipError_t e = dtype == RPY_F64
+                               ? hipFuncSetAttribute(
+                                         (const void*)sig_vjp_kernel<double, true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(sh))
+                               : hipFuncSetAttribute(
+                                         (const void*)sig_vjp_kernel<float, true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(sh));
+        if (e != hipSuccess)
+            return fail(RPY_ERR_HIP, "cannot reserve %zu bytes of LDS: %s", sh,
+                        hipGetErrorString(e));
+    }
+    if (dtype == RPY_F64) {
+        if (in_lds)
+            hipLaunchKernelGGL((sig_vjp_kernel<double, true>), g, blk, sh, st, (double*)grad,
+                               (const double*)increments, offsets, n_streams,
+                               (const double*)signatures, (const double*)grad_signatures,
+                               (double*)nullptr, b, pw);
+        else
+            hipLaunchKernelGGL((sig_vjp_kernel<double, false>), g, blk, 0, st, (double*)grad,
+                               (const double*)increments, offsets, n_streams,
+                               (const double*)signatures, (const double*)grad_signatures,
+                               (double*)scratch, b, pw);
+    } else {
+        if (in_lds)
+            hipLaunchKernelGGL((sig_vjp_kernel<float, true>), g, blk, sh, st, (float*)grad,
+                               (const float*)increments, offsets, n_streams,
+                               (const float*)signatures, (const float*)grad_signatures,
+                               (float*)nullptr, b, pw);
+        else
+            hipLaunchKernelGGL((sig_vjp_kernel<float, false>), g, blk, 0, st, (float*)grad,
+                               (const float*)increments, offsets, n_streams,
+                               (const float*)signatures, (const float*)grad_signatures,
+                               (float*)scratch, b, pw);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "vjp launch failed: %s", hipGetErrorString(err));
-    hipError_t ferr = hipFreeAsync(scratch, st);
-    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    if (scratch) {
+        hipError_t ferr = hipFreeAsync(scratch, st);
+        if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    }
     return rc;
 }
 

@@ -271,6 +271,16 @@ int rpy_debug_signature_host_emulation(double* out, const double* increments, in
     return signature_host_emulation(out, increments, length, width, depth, prefix);
 }
 
+int rpy_debug_vjp_host_emulation(double* grad, const double* increments, int64_t length,
+                                 int32_t width, int32_t depth, const double* signature,
+                                 const double* grad_signature)
+{
+    if (!grad || !signature || !grad_signature || (!increments && length > 0))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
+    return vjp_host_emulation(grad, increments, length, width, depth, signature,
+                              grad_signature);
+}
+
 // Prefix length the specialised signature kernel uses, 0 = generic kernel.
 int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth, int32_t is_f64)
 {
