@@ -262,22 +262,9 @@ int launch_persistent(void* out, const void* incr, const int64_t* begins, const 
     return RPY_OK;
 }
 
-// Prefix length per shape: the smallest P whose per-lane state fits the
-// register budget (about 160 VGPRs of state for float, 128 for double).
-constexpr int choose_p(int W, int D, int budget)
-{
-    int p = 1;
-    while (p < D) {
-        int sub = 0;
-        for (int j = p; j <= D; ++j) sub += int(ipow(W, j - p));
-        if (sub + p <= budget) break;
-        ++p;
-    }
-    return p;
-}
-
-constexpr int kBudgetF32 = 160;  // state values per lane (1 VGPR each)
-constexpr int kBudgetF64 = 64;   // state values per lane (2 VGPRs each)
+using chen::choose_p;
+using chen::kBudgetF32;
+using chen::kBudgetF64;
 
 struct Entry {
     int w, d, p32, p64;

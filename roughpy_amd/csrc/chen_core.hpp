@@ -51,6 +51,23 @@ RPY_HD inline __attribute__((always_inline)) void static_for(F&& f)
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// Prefix length per shape: the smallest P whose per-lane state fits the
+// register budget (about 160 VGPRs of state for float, 128 for double).
+constexpr int choose_p(int W, int D, int budget)
+{
+    int p = 1;
+    while (p < D) {
+        int sub = 0;
+        for (int j = p; j <= D; ++j) sub += int(ipow(W, j - p));
+        if (sub + p <= budget) break;
+        ++p;
+    }
+    return p;
+}
+
+constexpr int kBudgetF32 = 160;  // state values per lane (1 VGPR each)
+constexpr int kBudgetF64 = 64;   // state values per lane (2 VGPRs each)
+
 template <int W, int D, int P>
 struct Shape {
     static_assert(W >= 1 && D >= 1 && P >= 1 && P <= D, "bad shape");

@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 
 #include "dense_ops.hpp"
@@ -63,8 +65,6 @@ __global__ __launch_bounds__(kBlock) void ft_fma_kernel(
         T* o = out + e * os;
         const T* l = lhs + e * ls;
         const T* r = rhs + e * rs;
-        const T* L = l;
-        const T* R = r;
         if (Stage) {
             // levels below `top` are re-read by many outputs: stage them.
             for (int64_t i = threadIdx.x; i < b.db[top]; i += kBlock) {
@@ -72,25 +72,60 @@ __global__ __launch_bounds__(kBlock) void ft_fma_kernel(
                 sr[i] = r[i];
             }
             __syncthreads();
-            L = sl;
-            R = sr;
         }
         for (int od = ob.hi; od >= omin; --od) {
-            const int lmax = min(lb.hi, od - rb.lo);
-            const int lmin = max(lb.lo, od - rb.hi);
+            int lmax = min(lb.hi, od - rb.lo);
+            int lmin = max(lb.lo, od - rb.hi);
             if (lmax < lmin) continue;
             const uint32_t n = uint32_t(b.level_size(od));
-            for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
-                Acc acc = Acc(o[b.db[od] + idx]);
-                for (int ld = lmax; ld >= lmin; --ld) {
-                    const int rd = od - ld;
-                    const uint32_t i = fdiv(idx, pw.pw[rd]);
-                    const uint32_t j = idx - i * pw.pw[rd].d;
-                    const T lv = (ld < top) ? L[b.db[ld] + i] : l[b.db[ld] + i];
-                    const T rv = (rd < top) ? R[b.db[rd] + j] : r[b.db[rd] + j];
-                    acc += Acc(lv) * Acc(rv);
+            // With staging, only the two end terms of the top level read
+            // global memory (lhs_top * rhs_0 and lhs_0 * rhs_top); every other
+            // operand comes from LDS.  LDS and global reads use distinct
+            // pointers so each compiles to its own (non-generic) load.
+            const bool top_hi = Stage && od == top && lmax == top;
+            const bool top_lo = Stage && od == top && lmin == 0;
+            if (top_hi) --lmax;
+            if (top_lo) ++lmin;
+            // kU outputs per thread per pass: their global loads are issued
+            // together before any arithmetic, so HBM latency overlaps.
+            constexpr int kU = 4;
+            for (uint32_t base = threadIdx.x; base < n; base += kU * kBlock) {
+                Acc acc[kU];
+                T lt[kU], rt[kU];
+#pragma unroll
+                for (int q = 0; q < kU; ++q) {
+                    const uint32_t idx = base + q * kBlock;
+                    const bool ok = idx < n;
+                    acc[q] = ok ? Acc(o[b.db[od] + idx]) : Acc(0);
+                    lt[q] = (ok && top_hi) ? l[b.db[od] + idx] : T(0);
+                    rt[q] = (ok && top_lo) ? r[b.db[od] + idx] : T(0);
                 }
-                o[b.db[od] + idx] = T(acc);
+#pragma unroll
+                for (int q = 0; q < kU; ++q) {
+                    const uint32_t idx = base + q * kBlock;
+                    if (idx >= n) continue;
+                    if (top_hi) acc[q] += Acc(lt[q]) * Acc(sr[0]);
+                    for (int ld = lmax; ld >= lmin; --ld) {
+                        const int rd = od - ld;
+                        const uint32_t i = fdiv(idx, pw.pw[rd]);
+                        const uint32_t j = idx - i * pw.pw[rd].d;
+                        T lv, rv;
+                        if (Stage) {
+                            lv = sl[b.db[ld] + i];
+                            rv = sr[b.db[rd] + j];
+                        } else {
+                            lv = l[b.db[ld] + i];
+                            rv = r[b.db[rd] + j];
+                        }
+                        acc[q] += Acc(lv) * Acc(rv);
+                    }
+                    if (top_lo) acc[q] += Acc(sl[0]) * Acc(rt[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < kU; ++q) {
+                    const uint32_t idx = base + q * kBlock;
+                    if (idx < n) o[b.db[od] + idx] = T(acc[q]);
+                }
             }
         }
         if (threadIdx.x == 0 && ob.lo == 0 && lb.lo == 0 && rb.lo == 0)
@@ -121,23 +156,38 @@ __global__ __launch_bounds__(kBlock) void ft_inplace_mul_kernel(
             sr[i] = r[i];
         }
         __syncthreads();
-        const T r0 = r[0];
+        const T r0 = sr[0];
         for (int od = lb.hi; od >= lo; --od) {
             const int lmin = max(lb.lo, od - rb.hi);
-            const int lmax = od - max(1, rb.lo);
+            int lmax = od - max(1, rb.lo);
             const uint32_t n = uint32_t(b.level_size(od));
-            for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
-                const T old = (od < top) ? sl[b.db[od] + idx] : l[b.db[od] + idx];
-                T acc = (rb.lo == 0) ? old * r0 : T(0);
-                for (int ld = lmax; ld >= lmin; --ld) {
-                    const int rd = od - ld;
-                    const uint32_t i = fdiv(idx, pw.pw[rd]);
-                    const uint32_t j = idx - i * pw.pw[rd].d;
-                    const T rv = (rd < top) ? sr[b.db[rd] + j] : r[b.db[rd] + j];
-                    acc += sl[b.db[ld] + i] * rv;
+            // Levels below `top` come from LDS.  At the top level the old
+            // coefficient and the rhs_top operand of the ld = 0 term are read
+            // from global memory, through their own pointers.
+            const bool at_top = od >= top;
+            const bool top_lo = at_top && lmin == 0 && lmax >= 0;
+            const int lmin_lds = top_lo ? 1 : lmin;
+            // the two variants are separate code so each `old` load has a
+            // known address space
+            auto level = [&](auto at_top_c) {
+                constexpr bool kTop = decltype(at_top_c)::value;
+                for (uint32_t idx = threadIdx.x; idx < n; idx += kBlock) {
+                    const T old = kTop ? l[b.db[od] + idx] : sl[b.db[od] + idx];
+                    T acc = (rb.lo == 0) ? old * r0 : T(0);
+                    for (int ld = lmax; ld >= lmin_lds; --ld) {
+                        const int rd = od - ld;
+                        const uint32_t i = fdiv(idx, pw.pw[rd]);
+                        const uint32_t j = idx - i * pw.pw[rd].d;
+                        acc += sl[b.db[ld] + i] * sr[b.db[rd] + j];
+                    }
+                    if (top_lo) acc += sl[0] * r[b.db[od] + idx];
+                    l[b.db[od] + idx] = acc;
                 }
-                l[b.db[od] + idx] = acc;
-            }
+            };
+            if (at_top)
+                level(std::true_type{});
+            else
+                level(std::false_type{});
         }
         if (threadIdx.x == 0) {
             if (lb.lo == 0 && rb.lo == 0)

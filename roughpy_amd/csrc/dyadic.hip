@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "chen.hpp"
+#include "chen_bwd.hpp"
 #include "dense_ops.hpp"
 #include "kernel_util.hpp"
 #include "rpy_common.hpp"
@@ -209,6 +210,217 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_kernel(
     }
 }
 
+constexpr int kRunChunk = 64;  // fine-run rows staged per pass
+
+// Cover of a query split into [fine run][stored nodes][fine run]: pieces
+// finer than the stored levels sit only at the two ends of the cover, and the
+// fine pieces of one end are adjacent, so each end is one contiguous run of
+// rows.  Fills the row ranges of the runs and the index range of the stored
+// pieces; returns the piece count.
+RPY_FI int split_cover(int64_t A, int64_t B, int r, int J, const double* params, int64_t lo,
+                       int64_t hi, int64_t* pk, int* pn, int64_t* runs, int* coarse)
+{
+    const int np = dyadic_cover(A, B, r, pk, pn);
+    int c0 = 0;
+    while (c0 < np && pn[c0] > J) ++c0;
+    int c1 = c0;
+    while (c1 < np && pn[c1] <= J) ++c1;
+    // c1 == np or every piece after c1 is fine
+    auto start = [&](int p) { return std::ldexp(double(pk[p]), -pn[p]); };
+    auto end = [&](int p) { return std::ldexp(double(pk[p] + 1), -pn[p]); };
+    runs[0] = runs[1] = runs[2] = runs[3] = lo;
+    if (c0 > 0) {
+        runs[0] = lower_bound_dev(params, lo, hi, start(0));
+        runs[1] = lower_bound_dev(params, lo, hi, end(c0 - 1));
+    }
+    if (c1 < np) {
+        runs[2] = lower_bound_dev(params, lo, hi, start(c1));
+        runs[3] = lower_bound_dev(params, lo, hi, end(np - 1));
+    }
+    coarse[0] = c0;
+    coarse[1] = c1;
+    return np;
+}
+
+// Queries with the fine runs computed in registers: wave 0 takes the left
+// run, wave 1 the right run, each lane holding its prefix share of the run's
+// signature (chen_core.hpp) and stepping through the rows with no barrier;
+// then the workgroup multiplies left run, stored nodes and right run in LDS.
+template <typename T, int W, int D, int P>
+__global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
+        T* __restrict__ out, const T* __restrict__ cache, const T* __restrict__ incr,
+        const double* __restrict__ params, const int64_t* __restrict__ offsets, int64_t n_streams,
+        DevBasis b, Pow pw, int r, int J, const int64_t* __restrict__ q_stream,
+        const double* __restrict__ q_inf, const double* __restrict__ q_sup, int64_t n_queries)
+{
+    using Sh = chen::Shape<W, D, P>;
+    static_assert(Sh::NP <= 64, "a run fits one wave");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t tsize = b.size();
+    T* S = reinterpret_cast<T*>(smem);
+    T* N = S + tsize;
+    T* R = N + tsize;
+    __shared__ int64_t pk[kMaxPieces];
+    __shared__ int pn[kMaxPieces];
+    __shared__ int64_t runs[4];
+    __shared__ int coarse[2];
+    __shared__ int64_t qs;
+    __shared__ T XT[2 * kRunChunk * D * W];
+    const int wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+
+    for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const int64_t s = q_stream[q];
+            const bool valid = s >= 0 && s < n_streams;
+            int c[2] = {0, 0};
+            int64_t rr[4] = {0, 0, 0, 0};
+            if (valid) {
+                const double scale = std::ldexp(1.0, r);
+                double a = std::floor(q_inf[q] * scale), e = std::floor(q_sup[q] * scale);
+                a = std::min(std::max(a, 0.0), scale);
+                e = std::min(std::max(e, 0.0), scale);
+                if (a < e)
+                    split_cover(int64_t(a), int64_t(e), r, J, params, offsets[s], offsets[s + 1],
+                                pk, pn, rr, c);
+            }
+            for (int i = 0; i < 4; ++i) runs[i] = rr[i];
+            coarse[0] = c[0];
+            coarse[1] = c[1];
+            qs = valid ? s : 0;
+        }
+        __syncthreads();
+        // Fine runs: their rows are staged chunk by chunk in LDS as scaled
+        // tables (x_a / d) by the whole workgroup with coalesced loads; wave 0
+        // steps the left run, wave 1 the right run, the lane state staying in
+        // registers across chunks.
+        const int64_t l0 = runs[0], l1 = runs[1], rr0 = runs[2], rr1 = runs[3];
+        const int64_t longest = max(l1 - l0, rr1 - rr0);
+        int u[P];
+        {
+            int t = lane;
+            for (int j = P - 1; j >= 0; --j) {
+                u[j] = t % W;
+                t /= W;
+            }
+        }
+        chen::Lane<T, W, D, P> st;
+        st.init_identity();
+        for (int64_t c0 = 0; c0 < longest; c0 += kRunChunk) {
+            const int nl = int(min<int64_t>(kRunChunk, max<int64_t>(l1 - l0 - c0, 0)));
+            const int nr = int(min<int64_t>(kRunChunk, max<int64_t>(rr1 - rr0 - c0, 0)));
+            for (int e = threadIdx.x; e < (nl + nr) * D * W; e += kBlock) {
+                const int side = e >= nl * D * W;
+                const int f = side ? e - nl * D * W : e;
+                const int row = f / (D * W);
+                const int da = f - row * (D * W);
+                const int d = da / W + 1;
+                const int a = da - (d - 1) * W;
+                const int64_t grow = (side ? rr0 : l0) + c0 + row;
+                const T x = incr[grow * W + a];
+                XT[side * kRunChunk * D * W + f] = (d == 1) ? x : x * (T(1) / T(d));
+            }
+            __syncthreads();
+            if (wave < 2 && lane < Sh::NP) {
+                const int n = wave == 0 ? nl : nr;
+                const T* tab = XT + wave * kRunChunk * D * W;
+                for (int i = 0; i < n; ++i)
+                    st.step(chen::ScaledRow<T, W, D, P>{tab + i * D * W, u});
+            }
+            __syncthreads();
+        }
+        if (wave < 2 && lane < Sh::NP) {
+            T* dst = wave == 0 ? S : R;
+            chen::static_for<D - P + 1>([&](auto j0) {
+                constexpr int k = P + int(decltype(j0)::value);
+                constexpr int nk = int(ipow(W, k - P));
+                chen::static_for<nk>([&](auto v0) {
+                    constexpr int v = int(decltype(v0)::value);
+                    dst[b.db[k] + int64_t(lane) * nk + v] = st.sub[Sh::off(k) + v];
+                });
+            });
+            chen::static_for<P>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                constexpr int grp = int(ipow(W, P - k));
+                if (lane % grp == 0) dst[b.db[k] + lane / grp] = st.path[k];
+            });
+        }
+        __syncthreads();
+        const int64_t s = qs;
+        for (int p = coarse[0]; p < coarse[1]; ++p) {
+            const T* node = cache + node_offset(pn[p], n_streams, s, pk[p]) * tsize;
+            for (int64_t i = threadIdx.x; i < tsize; i += kBlock) N[i] = node[i];
+            __syncthreads();
+            wg_mul_inplace(S, N, b, pw);
+        }
+        if (runs[3] > runs[2]) wg_mul_inplace(S, R, b, pw);
+        T* o = out + q * tsize;
+        for (int64_t i = threadIdx.x; i < tsize; i += kBlock) o[i] = S[i];
+        __syncthreads();
+    }
+}
+
+size_t device_lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 64 * 1024;
+    return size_t(v);
+}
+
+// Launches the fast query kernel if its LDS (three tensors plus the staged
+// run tables) fits the device; returns false to fall back.
+template <typename T, int W, int D, int P>
+bool launch_query_fast(void* out, const void* cache, const void* incr, const double* params,
+                       const int64_t* offsets, int64_t n_streams, const DevBasis& b,
+                       const Pow& pw, int r, int J, const int64_t* q_stream, const double* q_inf,
+                       const double* q_sup, int64_t n_queries, uint32_t grid, hipStream_t st)
+{
+    const size_t dyn = 3 * size_t(b.size()) * sizeof(T);
+    const size_t stat = 2 * size_t(kRunChunk) * D * W * sizeof(T) + 1024;
+    if (dyn + stat > device_lds_limit()) return false;
+    if (dyn + stat > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)) != hipSuccess)
+        return false;
+    hipLaunchKernelGGL((dyadic_query_fast_kernel<T, W, D, P>), dim3(grid), dim3(kBlock), dyn, st,
+                       (T*)out, (const T*)cache, (const T*)incr, params, offsets, n_streams, b,
+                       pw, r, J, q_stream, q_inf, q_sup, n_queries);
+    return true;
+}
+
+using QueryLauncher = bool (*)(void*, const void*, const void*, const double*, const int64_t*,
+                               int64_t, const DevBasis&, const Pow&, int, int, const int64_t*,
+                               const double*, const double*, int64_t, uint32_t, hipStream_t);
+
+template <typename T, int W, int D>
+constexpr QueryLauncher query_entry()
+{
+    constexpr int P = chen::choose_p(W, D, sizeof(T) == 4 ? chen::kBudgetF32 : chen::kBudgetF64);
+    if constexpr (ipow(W, P) <= 64)
+        return &launch_query_fast<T, W, D, P>;
+    else
+        return nullptr;
+}
+
+// Shapes whose prefix split fits one wave, float32 and float64.
+template <typename T>
+QueryLauncher find_query_fast(int w, int d)
+{
+#define RPY_Q(W, D) \
+    if (w == W && d == D) return query_entry<T, W, D>();
+    RPY_Q(2, 2) RPY_Q(2, 3) RPY_Q(2, 4) RPY_Q(2, 5) RPY_Q(2, 6) RPY_Q(2, 7) RPY_Q(2, 8)
+    RPY_Q(3, 2) RPY_Q(3, 3) RPY_Q(3, 4) RPY_Q(3, 5) RPY_Q(3, 6)
+    RPY_Q(4, 2) RPY_Q(4, 3) RPY_Q(4, 4) RPY_Q(4, 5) RPY_Q(4, 6)
+    RPY_Q(5, 2) RPY_Q(5, 3) RPY_Q(5, 4) RPY_Q(5, 5)
+    RPY_Q(6, 2) RPY_Q(6, 3) RPY_Q(6, 4)
+    RPY_Q(7, 2) RPY_Q(7, 3) RPY_Q(7, 4)
+    RPY_Q(8, 2) RPY_Q(8, 3) RPY_Q(8, 4)
+#undef RPY_Q
+    return nullptr;
+}
+
 uint32_t blocks_for(int64_t n)
 {
     return uint32_t(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65536)));
@@ -309,12 +521,19 @@ int dyadic_cache_query(void* out, const void* cache, const void* increments, con
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     const int64_t tsize = b.size();
     const size_t es = dtype == RPY_F64 ? 8 : 4;
-    const size_t lds = 2 * size_t(tsize) * es;
-    if (lds > 64 * 1024)
-        return fail(RPY_ERR_UNSUPPORTED, "queries need %zu bytes of LDS (limit 65536)", lds);
     const Pow pw = make_pow(width, depth);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const uint32_t grid = uint32_t(std::min<int64_t>(n_queries, 16384));
+    const QueryLauncher fast = dtype == RPY_F64 ? find_query_fast<double>(width, depth)
+                                                : find_query_fast<float>(width, depth);
+    if (fast && fast(out, cache, increments, params, offsets, n_streams, b, pw, resolution,
+                     levels, q_stream, q_inf, q_sup, n_queries, grid, st)) {
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
+    const size_t lds = 2 * size_t(tsize) * es;
+    if (lds > 64 * 1024)
+        return fail(RPY_ERR_UNSUPPORTED, "queries need %zu bytes of LDS (limit 65536)", lds);
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(dyadic_query_kernel<double>, dim3(grid), dim3(kBlock), lds, st,
                            (double*)out, (const double*)cache, (const double*)increments, params,

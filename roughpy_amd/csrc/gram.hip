@@ -6,8 +6,18 @@
 // s = hi + lo with hi = bf16(s), lo = bf16(s - hi), and
 //   G ~= Hi Hi^T + Hi Lo^T + Lo Hi^T
 // is accumulated in float32 with v_mfma_f32_32x32x16_bf16 (the Lo Lo^T term
-// is below float32 resolution).  G is symmetric, so only upper-triangle
-// 64x64 tiles are computed and mirrored.
+// is below float32 resolution).
+//
+// Tiling.  G is symmetric: only the upper-triangle 128 x 128 output tiles
+// are computed.  A workgroup (4 waves, each a 64 x 64 quarter = 2 x 2 MFMA
+// blocks) owns one tile and one slice of k.  Per 32-wide k step the four
+// operand tiles (rows of the tile's two row blocks, hi and lo) are staged in
+// LDS, double-buffered, with the next step's global loads in flight while
+// the current step's 24 MFMAs per wave run.  When the tiles alone do not
+// fill the chip, k is split; the slices' partial tiles go to a workspace and
+// a second kernel sums them in a fixed order (deterministic) and writes the
+// tile and its mirror (through an LDS transpose, so both writes coalesce).
+// With one slice the tile kernel writes G itself.
 //
 // MFMA 32x32x16 bf16 operand layout (wave64): lane l holds 8 consecutive k
 // of row (l % 32): A[l % 32][8 (l / 32) + 0..7], and likewise for B^T; the
@@ -28,8 +38,13 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kTile = 64;   // output tile per workgroup (2 x 2 waves of 32 x 32)
-constexpr int kK = 16;      // k per MFMA
+constexpr int kBT = 128;             // output tile edge
+constexpr int kBK = 32;              // k per LDS stage
+constexpr int kRow = kBK + 8;        // LDS row stride in bf16 (80 B: 16 B reads conflict-free)
+constexpr int kMat = kBT * kRow;     // one operand tile in LDS (bf16 elements)
+constexpr int kStage = 4 * kMat;     // A hi, A lo, B hi, B lo
+constexpr size_t kLdsBytes = size_t(2) * kStage * sizeof(uint16_t);  // 80 KiB
+constexpr int kTileElems = kBT * kBT;
 
 // bf16 round-to-nearest-even of a float, as bits
 RPY_FI uint16_t to_bf16_bits(float f)
@@ -43,76 +58,186 @@ RPY_FI uint16_t to_bf16_bits(float f)
 RPY_FI float bf16_bits_to_float(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 // hi/lo split into row-major (n x kp) bf16 arrays, zero padded to kp.
+// grid.y walks rows, grid.x and the block walk columns.
 __global__ void split_kernel(const float* __restrict__ S, int64_t n, int64_t k, int64_t ld,
                              int64_t kp, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo)
 {
-    const int64_t total = n * kp;
-    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
-         t += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t r = t / kp;
-        const int64_t c = t - r * kp;
-        const float v = c < k ? S[r * ld + c] : 0.0f;
-        const uint16_t h = to_bf16_bits(v);
-        hi[t] = h;
-        lo[t] = to_bf16_bits(v - bf16_bits_to_float(h));
-    }
-}
-
-RPY_FI bf16x8 load8(const uint16_t* p)
-{
-    const uint4 v = *reinterpret_cast<const uint4*>(p);
-    return __builtin_bit_cast(bf16x8, v);
-}
-
-// One workgroup = one upper-triangle 64x64 tile; wave (wr, wc) owns the
-// 32x32 sub-tile at (32 wr, 32 wc).
-__global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ hi,
-                                                   const uint16_t* __restrict__ lo, int64_t n,
-                                                   int64_t kp, float* __restrict__ G,
-                                                   int64_t ldg, int tiles)
-{
-    // map the linear block id to (bi <= bj)
-    int64_t t = blockIdx.x;
-    int bi = 0;
-    while (t >= tiles - bi) {
-        t -= tiles - bi;
-        ++bi;
-    }
-    const int bj = bi + int(t);
-    const int wave = threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
-    // the lower sub-tile of a diagonal tile is the mirror of the upper one
-    if (bi == bj && wave == 2) return;
-    const int64_t row0 = int64_t(bi) * kTile + 32 * (wave / 2);
-    const int64_t col0 = int64_t(bj) * kTile + 32 * (wave % 2);
-    const int64_t ra = row0 + lane % 32;  // A operand row of this lane
-    const int64_t rb = col0 + lane % 32;  // B operand row (of S) of this lane
-    const int koff = 8 * (lane / 32);
-    const bool va = ra < n, vb = rb < n;
-    const uint16_t* ah = hi + (va ? ra : 0) * kp + koff;
-    const uint16_t* al = lo + (va ? ra : 0) * kp + koff;
-    const uint16_t* bh = hi + (vb ? rb : 0) * kp + koff;
-    const uint16_t* bl = lo + (vb ? rb : 0) * kp + koff;
-    f32x16 acc = {};
-    const bf16x8 zero = {};
-    for (int64_t k = 0; k < kp; k += kK) {
-        const bf16x8 a_h = va ? load8(ah + k) : zero;
-        const bf16x8 a_l = va ? load8(al + k) : zero;
-        const bf16x8 b_h = vb ? load8(bh + k) : zero;
-        const bf16x8 b_l = vb ? load8(bl + k) : zero;
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, b_h, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, b_l, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l, b_h, acc, 0, 0, 0);
-    }
-    const int64_t j = col0 + lane % 32;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int64_t i = row0 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        // on diagonal tiles only i <= j is kept, so G is exactly symmetric
-        if (i < n && j < n && (bi != bj || i <= j)) {
-            G[i * ldg + j] = acc[r];
-            G[j * ldg + i] = acc[r];
+    for (int64_t r = blockIdx.y; r < n; r += gridDim.y) {
+        for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < kp;
+             c += int64_t(gridDim.x) * blockDim.x) {
+            const float v = c < k ? S[r * ld + c] : 0.0f;
+            const uint16_t h = to_bf16_bits(v);
+            hi[r * kp + c] = h;
+            lo[r * kp + c] = to_bf16_bits(v - bf16_bits_to_float(h));
         }
+    }
+}
+
+RPY_FI bf16x8 lds8(const uint16_t* p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+// upper-triangle tile t -> (bi, bj), bi <= bj, row-major over the triangle
+RPY_FI void tile_coords(int t, int tiles, int& bi, int& bj)
+{
+    int i = 0;
+    while (t >= tiles - i) {
+        t -= tiles - i;
+        ++i;
+    }
+    bi = i;
+    bj = i + t;
+}
+
+template <bool Direct>
+__global__ __launch_bounds__(256) void gram_tile_kernel(
+        const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo, int64_t n, int64_t kp,
+        int tiles, int ntiles, int splits, float* __restrict__ out, int64_t ldg)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint16_t* sm = reinterpret_cast<uint16_t*>(smem_raw);
+    const int t = blockIdx.x;
+    const int split = blockIdx.y;
+    int bi, bj;
+    tile_coords(t, tiles, bi, bj);
+    const int tid = threadIdx.x;
+    const int wave = tid / 64;
+    const int lane = tid % 64;
+    const int wr = wave / 2, wc = wave % 2;
+
+    const int64_t steps = kp / kBK;
+    const int64_t s0 = steps * split / splits;
+    const int64_t s1 = steps * (split + 1) / splits;
+
+    // global -> register staging: 8 chunks of 8 bf16 per thread per stage;
+    // chunk c: matrix c / 2, row (tid + 256 (c % 2)) / 4, k chunk tid % 4
+    uint4 stage_regs[8];
+    auto load_stage = [&](int64_t step) {
+        const int64_t k0 = step * kBK;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int m = c / 2;
+            const int id = tid + 256 * (c % 2);
+            const int row = id / 4;
+            const int kc = id % 4;
+            const uint16_t* srcm = (m % 2 == 0) ? hi : lo;
+            const int64_t grow = int64_t(m < 2 ? bi : bj) * kBT + row;
+            if (grow < n)
+                stage_regs[c] = *reinterpret_cast<const uint4*>(srcm + grow * kp + k0 + kc * 8);
+            else
+                stage_regs[c] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int m = c / 2;
+            const int id = tid + 256 * (c % 2);
+            const int row = id / 4;
+            const int kc = id % 4;
+            *reinterpret_cast<uint4*>(sm + buf * kStage + m * kMat + row * kRow + kc * 8) =
+                    stage_regs[c];
+        }
+    };
+
+    f32x16 acc[2][2] = {};
+    int buf = 0;
+    if (s0 < s1) {
+        load_stage(s0);
+        store_stage(0);
+    }
+    __syncthreads();
+    for (int64_t step = s0; step < s1; ++step) {
+        const bool more = step + 1 < s1;
+        if (more) load_stage(step + 1);
+        const uint16_t* base = sm + buf * kStage;
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            const int koff = kk + 8 * (lane / 32);
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int arow = wr * 64 + r * 32 + lane % 32;
+                const int brow = wc * 64 + r * 32 + lane % 32;
+                ah[r] = lds8(base + 0 * kMat + arow * kRow + koff);
+                al[r] = lds8(base + 1 * kMat + arow * kRow + koff);
+                bh[r] = lds8(base + 2 * kMat + brow * kRow + koff);
+                bl[r] = lds8(base + 3 * kMat + brow * kRow + koff);
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[r][c],
+                                                                         0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[r][c],
+                                                                         0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[r][c],
+                                                                         0, 0, 0);
+                }
+        }
+        if (more) store_stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // epilogue
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int li = wr * 64 + r * 32 + 8 * (e / 4) + 4 * (lane / 32) + (e % 4);
+                const int lj = wc * 64 + c * 32 + lane % 32;
+                const float v = acc[r][c][e];
+                if (Direct) {
+                    const int64_t i = int64_t(bi) * kBT + li;
+                    const int64_t j = int64_t(bj) * kBT + lj;
+                    // on diagonal tiles only i <= j is kept, so G is exactly symmetric
+                    if (i < n && j < n && (bi != bj || i <= j)) {
+                        out[i * ldg + j] = v;
+                        out[j * ldg + i] = v;
+                    }
+                } else {
+                    out[(int64_t(split) * ntiles + t) * kTileElems + li * kBT + lj] = v;
+                }
+            }
+}
+
+// Sums the k-slices of one 32 x 32 block of an upper-triangle tile and
+// writes it and its mirror.  grid: (tiles, 16 blocks per tile).
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const float* __restrict__ ws,
+                                                          int tiles, int ntiles, int splits,
+                                                          int64_t n, float* __restrict__ G,
+                                                          int64_t ldg)
+{
+    __shared__ float blk[32][33];
+    const int t = blockIdx.x;
+    int bi, bj;
+    tile_coords(t, tiles, bi, bj);
+    const int br = blockIdx.y / 4, bc = blockIdx.y % 4;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = ty + 8 * q;
+        const int li = br * 32 + r, lj = bc * 32 + tx;
+        float v = 0.0f;
+        for (int s = 0; s < splits; ++s)
+            v += ws[(int64_t(s) * ntiles + t) * kTileElems + li * kBT + lj];
+        blk[r][tx] = v;
+        const int64_t i = int64_t(bi) * kBT + li, j = int64_t(bj) * kBT + lj;
+        if (i < n && j < n && (bi != bj || i <= j)) G[i * ldg + j] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = ty + 8 * q;  // column of the block = row of the mirror
+        const int li = br * 32 + tx, lj = bc * 32 + c;
+        const int64_t i = int64_t(bi) * kBT + li, j = int64_t(bj) * kBT + lj;
+        // mirror element G[j][i] = block[tx][c]
+        if (i < n && j < n && (bi != bj || i < j)) G[j * ldg + i] = blk[tx][c];
     }
 }
 
@@ -125,24 +250,58 @@ int gram_bf16x3(float* G, const float* S, int64_t n, int64_t k, int64_t lds, int
     if (n == 0) return RPY_OK;
     if (!G || !S) return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
     if (lds < k || ldg < n) return fail(RPY_ERR_SHAPE, "leading dimension too small");
+    const int64_t tiles64 = (n + kBT - 1) / kBT;
+    const int64_t ntiles64 = tiles64 * (tiles64 + 1) / 2;
+    if (ntiles64 > 0x7fffffffLL) return fail(RPY_ERR_SHAPE, "Gram matrix too large");
+    const int tiles = int(tiles64);
+    const int ntiles = int(ntiles64);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int64_t kp = std::max<int64_t>(kK, (k + kK - 1) / kK * kK);
-    uint16_t* buf = nullptr;
-    RPY_HIP_CHECK(hipMallocAsync((void**)&buf, size_t(2 * n * kp) * sizeof(uint16_t), st));
-    const int64_t total = n * kp;
-    const uint32_t blocks = uint32_t(std::min<int64_t>((total + 255) / 256, 65536));
-    hipLaunchKernelGGL(split_kernel, dim3(blocks), dim3(256), 0, st, S, n, k, lds, kp, buf,
-                       buf + n * kp);
+    const int64_t kp = std::max<int64_t>(kBK, (k + kBK - 1) / kBK * kBK);
+    const int64_t steps = kp / kBK;
+    // split k until about four workgroups per CU are busy (256 CUs), keeping
+    // at least four k steps per slice
+    int splits = 1;
+    while (splits < 16 && int64_t(ntiles) * splits < 1024 && steps / (2 * splits) >= 4)
+        splits *= 2;
+
+    const size_t split_bytes = size_t(2 * n * kp) * sizeof(uint16_t);
+    const size_t ws_bytes =
+            splits > 1 ? size_t(splits) * size_t(ntiles) * kTileElems * sizeof(float) : 0;
+    unsigned char* buf = nullptr;
+    RPY_HIP_CHECK(hipMallocAsync((void**)&buf, split_bytes + ws_bytes, st));
+    uint16_t* hi = reinterpret_cast<uint16_t*>(buf);
+    uint16_t* lo = hi + n * kp;
+    float* ws = splits > 1 ? reinterpret_cast<float*>(buf + split_bytes) : nullptr;
+
     int rc = RPY_OK;
+    const dim3 sgrid = dim3(uint32_t(std::min<int64_t>((kp + 255) / 256, 64)),
+                            uint32_t(std::min<int64_t>(n, 65535)));
+    hipLaunchKernelGGL(split_kernel, sgrid, dim3(256), 0, st, S, n, k, lds, kp, hi, lo);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "split launch failed: %s", hipGetErrorString(err));
     if (!rc) {
-        const int tiles = int((n + kTile - 1) / kTile);
-        const int64_t nblocks = int64_t(tiles) * (tiles + 1) / 2;
-        hipLaunchKernelGGL(gram_kernel, dim3(uint32_t(nblocks)), dim3(256), 0, st, buf, buf + n * kp,
-                           n, kp, G, ldg, tiles);
-        err = hipGetLastError();
-        if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "gram launch failed: %s", hipGetErrorString(err));
+        const dim3 grid = dim3(uint32_t(ntiles), uint32_t(splits));
+        if (splits == 1) {
+            err = hipFuncSetAttribute((const void*)gram_tile_kernel<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsBytes));
+            if (err == hipSuccess)
+                hipLaunchKernelGGL(gram_tile_kernel<true>, grid, dim3(256), kLdsBytes, st, hi, lo,
+                                   n, kp, tiles, ntiles, 1, G, ldg);
+        } else {
+            err = hipFuncSetAttribute((const void*)gram_tile_kernel<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsBytes));
+            if (err == hipSuccess) {
+                hipLaunchKernelGGL(gram_tile_kernel<false>, grid, dim3(256), kLdsBytes, st, hi,
+                                   lo, n, kp, tiles, ntiles, splits, ws, int64_t(0));
+                err = hipGetLastError();
+                if (err == hipSuccess)
+                    hipLaunchKernelGGL(gram_reduce_kernel, dim3(uint32_t(ntiles), 16), dim3(256),
+                                       0, st, ws, tiles, ntiles, splits, n, G, ldg);
+            }
+        }
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err != hipSuccess)
+            rc = fail(RPY_ERR_HIP, "gram launch failed: %s", hipGetErrorString(err));
     }
     hipError_t ferr = hipFreeAsync(buf, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");

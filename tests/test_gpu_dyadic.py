@@ -77,3 +77,23 @@ def test_cache_benchmark_shape_sample(gpu):
         sel = (params[offs[s]:offs[s + 1]] >= a[q]) & (params[offs[s]:offs[s + 1]] < b[q])
         ref = O.signature(inc64[offs[s]:offs[s + 1]][sel], w, d)
         assert O.level_errors(out[q], ref, w, d).max() <= 2e-5
+
+
+def test_queries_on_the_workgroup_path(gpu, rng):
+    """Width 5, depth 5 in float64 has no one-wave prefix split (125 lanes),
+    so its queries take the all-workgroup kernel; same oracle check."""
+    w, d, r, J = 5, 5, 8, 3
+    inc, params, offs, lens = _streams(rng, 3, w, 20, 200, np.float64)
+    cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J,
+                                    dtype=np.float64)
+    nq = 24
+    sid = rng.integers(0, 3, size=nq)
+    a = np.floor(rng.uniform(0, 1, size=nq) * 2 ** r) / 2 ** r
+    b = np.floor(rng.uniform(0, 1, size=nq) * 2 ** r) / 2 ** r
+    a, b = np.minimum(a, b), np.maximum(a, b)
+    out = cache.query(sid, a, b).cpu().numpy()
+    for q in range(nq):
+        s = sid[q]
+        sel = (params[offs[s]:offs[s + 1]] >= a[q]) & (params[offs[s]:offs[s + 1]] < b[q])
+        ref = O.signature(inc[offs[s]:offs[s + 1]][sel], w, d)
+        assert O.level_errors(out[q], ref, w, d).max() <= 1e-12, q
