@@ -210,7 +210,16 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_kernel(
     }
 }
 
-constexpr int kRunChunk = 64;  // fine-run rows staged per pass
+constexpr int kQueryChunk = 24;  // fine-run rows staged per pass and side
+
+// offset of level k in a dense tensor of width W
+template <int W>
+constexpr int dbeg(int k)
+{
+    int s = 0;
+    for (int j = 0; j < k; ++j) s += int(ipow(W, j));
+    return s;
+}
 
 // Cover of a query split into [fine run][stored nodes][fine run]: pieces
 // finer than the stored levels sit only at the two ends of the cover, and the
@@ -242,32 +251,72 @@ RPY_FI int split_cover(int64_t A, int64_t B, int r, int J, const double* params,
     return np;
 }
 
-// Queries with the fine runs computed in registers: wave 0 takes the left
-// run, wave 1 the right run, each lane holding its prefix share of the run's
-// signature (chen_core.hpp) and stepping through the rows with no barrier;
-// then the workgroup multiplies left run, stored nodes and right run in LDS.
+// Unrolled S <- S (x) N for a compile-time shape, both tensors in LDS: for
+// each output level (top-down, barrier per level) every thread takes outputs
+// idx = tid + 256 q and sums lhs degree m-i times rhs degree i, i = 0..m
+// (the reference order, lhs degree high to low), with constant divisors.
+template <typename T, int W, int D>
+RPY_FI void wg_mul_inplace_t(T* S, const T* N)
+{
+    chen::static_for<D>([&](auto i0) {
+        constexpr int m = D - int(decltype(i0)::value);
+        constexpr int n = int(ipow(W, m));
+        constexpr int dbm = dbeg<W>(m);
+        for (int idx = threadIdx.x; idx < n; idx += kBlock) {
+            T lv[m + 1], rv[m + 1];
+            chen::static_for<m + 1>([&](auto k0) {
+                constexpr int i = int(decltype(k0)::value);
+                constexpr int wi = int(ipow(W, i));
+                const int pre = idx / wi;
+                const int suf = idx - pre * wi;
+                lv[i] = S[dbeg<W>(m - i) + pre];
+                rv[i] = N[dbeg<W>(i) + suf];
+            });
+            T acc = T(0);
+            chen::static_for<m + 1>([&](auto k0) {
+                constexpr int i = int(decltype(k0)::value);
+                acc += lv[i] * rv[i];
+            });
+            S[dbm + idx] = acc;
+        }
+        __syncthreads();
+    });
+    if (threadIdx.x == 0) S[0] = S[0] * N[0];
+    __syncthreads();
+}
+
+// Queries for shapes whose prefix split has at most 128 lanes.  Per query:
+// 1. thread 0 splits the cover into [left fine run][stored nodes][right run];
+// 2. the rows of both runs are staged chunk by chunk in LDS as scaled tables
+//    (x_a / d) by the whole workgroup; threads 0..127 step the left run and
+//    128..255 the right run, each lane holding its prefix share of the run's
+//    signature in registers (chen_core.hpp), then write it to S and R;
+// 3. S is multiplied by each stored node (the next node's coefficients are
+//    loaded into registers while the current product runs) and by R.
 template <typename T, int W, int D, int P>
 __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
         T* __restrict__ out, const T* __restrict__ cache, const T* __restrict__ incr,
         const double* __restrict__ params, const int64_t* __restrict__ offsets, int64_t n_streams,
-        DevBasis b, Pow pw, int r, int J, const int64_t* __restrict__ q_stream,
-        const double* __restrict__ q_inf, const double* __restrict__ q_sup, int64_t n_queries)
+        int r, int J, const int64_t* __restrict__ q_stream, const double* __restrict__ q_inf,
+        const double* __restrict__ q_sup, int64_t n_queries)
 {
     using Sh = chen::Shape<W, D, P>;
-    static_assert(Sh::NP <= 64, "a run fits one wave");
+    static_assert(Sh::NP <= 128, "a run fits two waves");
+    constexpr int kT = dbeg<W>(D + 1);                  // tensor size
+    constexpr int kPer = (kT + kBlock - 1) / kBlock;    // node values per thread
+    constexpr int kDW = D * W;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int64_t tsize = b.size();
     T* S = reinterpret_cast<T*>(smem);
-    T* N = S + tsize;
-    T* R = N + tsize;
+    T* N = S + kT;
+    T* R = N + kT;
     __shared__ int64_t pk[kMaxPieces];
     __shared__ int pn[kMaxPieces];
     __shared__ int64_t runs[4];
     __shared__ int coarse[2];
     __shared__ int64_t qs;
-    __shared__ T XT[2 * kRunChunk * D * W];
-    const int wave = threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
+    __shared__ T XT[2 * kQueryChunk * kDW];
+    const int half = threadIdx.x / 128;   // 0: left run, 1: right run
+    const int q_lane = threadIdx.x % 128;
 
     for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
         if (threadIdx.x == 0) {
@@ -290,15 +339,25 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
             qs = valid ? s : 0;
         }
         __syncthreads();
-        // Fine runs: their rows are staged chunk by chunk in LDS as scaled
-        // tables (x_a / d) by the whole workgroup with coalesced loads; wave 0
-        // steps the left run, wave 1 the right run, the lane state staying in
-        // registers across chunks.
+        const int64_t s = qs;
+        const int c0 = coarse[0], c1 = coarse[1];
+        // first stored node: its loads overlap the fine runs
+        T pre[kPer];
+        auto load_node = [&](int p) {
+            const T* node = cache + node_offset(pn[p], n_streams, s, pk[p]) * kT;
+            chen::static_for<kPer>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + k * kBlock;
+                if (i < kT) pre[k] = node[i];
+            });
+        };
+        if (c0 < c1) load_node(c0);
+
         const int64_t l0 = runs[0], l1 = runs[1], rr0 = runs[2], rr1 = runs[3];
         const int64_t longest = max(l1 - l0, rr1 - rr0);
         int u[P];
         {
-            int t = lane;
+            int t = q_lane;
             for (int j = P - 1; j >= 0; --j) {
                 u[j] = t % W;
                 t /= W;
@@ -306,56 +365,59 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
         }
         chen::Lane<T, W, D, P> st;
         st.init_identity();
-        for (int64_t c0 = 0; c0 < longest; c0 += kRunChunk) {
-            const int nl = int(min<int64_t>(kRunChunk, max<int64_t>(l1 - l0 - c0, 0)));
-            const int nr = int(min<int64_t>(kRunChunk, max<int64_t>(rr1 - rr0 - c0, 0)));
-            for (int e = threadIdx.x; e < (nl + nr) * D * W; e += kBlock) {
-                const int side = e >= nl * D * W;
-                const int f = side ? e - nl * D * W : e;
-                const int row = f / (D * W);
-                const int da = f - row * (D * W);
+        for (int64_t cs = 0; cs < longest; cs += kQueryChunk) {
+            const int nl = int(min<int64_t>(kQueryChunk, max<int64_t>(l1 - l0 - cs, 0)));
+            const int nr = int(min<int64_t>(kQueryChunk, max<int64_t>(rr1 - rr0 - cs, 0)));
+            for (int e = threadIdx.x; e < (nl + nr) * kDW; e += kBlock) {
+                const int side = e >= nl * kDW;
+                const int f = side ? e - nl * kDW : e;
+                const int row = f / kDW;
+                const int da = f - row * kDW;
                 const int d = da / W + 1;
                 const int a = da - (d - 1) * W;
-                const int64_t grow = (side ? rr0 : l0) + c0 + row;
+                const int64_t grow = (side ? rr0 : l0) + cs + row;
                 const T x = incr[grow * W + a];
-                XT[side * kRunChunk * D * W + f] = (d == 1) ? x : x * (T(1) / T(d));
+                XT[side * kQueryChunk * kDW + f] = (d == 1) ? x : x * (T(1) / T(d));
             }
             __syncthreads();
-            if (wave < 2 && lane < Sh::NP) {
-                const int n = wave == 0 ? nl : nr;
-                const T* tab = XT + wave * kRunChunk * D * W;
+            if (q_lane < Sh::NP) {
+                const int n = half == 0 ? nl : nr;
+                const T* tab = XT + half * kQueryChunk * kDW;
                 for (int i = 0; i < n; ++i)
-                    st.step(chen::ScaledRow<T, W, D, P>{tab + i * D * W, u});
+                    st.step(chen::ScaledRow<T, W, D, P>{tab + i * kDW, u});
             }
             __syncthreads();
         }
-        if (wave < 2 && lane < Sh::NP) {
-            T* dst = wave == 0 ? S : R;
+        if (q_lane < Sh::NP) {
+            T* dst = half == 0 ? S : R;
             chen::static_for<D - P + 1>([&](auto j0) {
                 constexpr int k = P + int(decltype(j0)::value);
                 constexpr int nk = int(ipow(W, k - P));
                 chen::static_for<nk>([&](auto v0) {
                     constexpr int v = int(decltype(v0)::value);
-                    dst[b.db[k] + int64_t(lane) * nk + v] = st.sub[Sh::off(k) + v];
+                    dst[dbeg<W>(k) + q_lane * nk + v] = st.sub[Sh::off(k) + v];
                 });
             });
             chen::static_for<P>([&](auto k0) {
                 constexpr int k = int(decltype(k0)::value);
                 constexpr int grp = int(ipow(W, P - k));
-                if (lane % grp == 0) dst[b.db[k] + lane / grp] = st.path[k];
+                if (q_lane % grp == 0) dst[dbeg<W>(k) + q_lane / grp] = st.path[k];
             });
         }
-        __syncthreads();
-        const int64_t s = qs;
-        for (int p = coarse[0]; p < coarse[1]; ++p) {
-            const T* node = cache + node_offset(pn[p], n_streams, s, pk[p]) * tsize;
-            for (int64_t i = threadIdx.x; i < tsize; i += kBlock) N[i] = node[i];
-            __syncthreads();
-            wg_mul_inplace(S, N, b, pw);
+        for (int p = c0; p < c1; ++p) {
+            chen::static_for<kPer>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + k * kBlock;
+                if (i < kT) N[i] = pre[k];
+            });
+            __syncthreads();  // N staged; S and R written by the run lanes
+            if (p + 1 < c1) load_node(p + 1);
+            wg_mul_inplace_t<T, W, D>(S, N);
         }
-        if (runs[3] > runs[2]) wg_mul_inplace(S, R, b, pw);
-        T* o = out + q * tsize;
-        for (int64_t i = threadIdx.x; i < tsize; i += kBlock) o[i] = S[i];
+        if (c0 >= c1) __syncthreads();  // S and R written by the run lanes
+        if (rr1 > rr0) wg_mul_inplace_t<T, W, D>(S, R);
+        T* o = out + q * kT;
+        for (int i = threadIdx.x; i < kT; i += kBlock) o[i] = S[i];
         __syncthreads();
     }
 }
@@ -373,38 +435,51 @@ size_t device_lds_limit()
 // run tables) fits the device; returns false to fall back.
 template <typename T, int W, int D, int P>
 bool launch_query_fast(void* out, const void* cache, const void* incr, const double* params,
-                       const int64_t* offsets, int64_t n_streams, const DevBasis& b,
-                       const Pow& pw, int r, int J, const int64_t* q_stream, const double* q_inf,
-                       const double* q_sup, int64_t n_queries, uint32_t grid, hipStream_t st)
+                       const int64_t* offsets, int64_t n_streams, int r, int J,
+                       const int64_t* q_stream, const double* q_inf, const double* q_sup,
+                       int64_t n_queries, uint32_t grid, hipStream_t st)
 {
-    const size_t dyn = 3 * size_t(b.size()) * sizeof(T);
-    const size_t stat = 2 * size_t(kRunChunk) * D * W * sizeof(T) + 1024;
+    const size_t dyn = 3 * size_t(dbeg<W>(D + 1)) * sizeof(T);
+    const size_t stat = 2 * size_t(kQueryChunk) * D * W * sizeof(T) + 1024;
     if (dyn + stat > device_lds_limit()) return false;
     if (dyn + stat > 64 * 1024 &&
         hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)) != hipSuccess)
         return false;
     hipLaunchKernelGGL((dyadic_query_fast_kernel<T, W, D, P>), dim3(grid), dim3(kBlock), dyn, st,
-                       (T*)out, (const T*)cache, (const T*)incr, params, offsets, n_streams, b,
-                       pw, r, J, q_stream, q_inf, q_sup, n_queries);
+                       (T*)out, (const T*)cache, (const T*)incr, params, offsets, n_streams, r,
+                       J, q_stream, q_inf, q_sup, n_queries);
     return true;
 }
 
 using QueryLauncher = bool (*)(void*, const void*, const void*, const double*, const int64_t*,
-                               int64_t, const DevBasis&, const Pow&, int, int, const int64_t*,
-                               const double*, const double*, int64_t, uint32_t, hipStream_t);
+                               int64_t, int, int, const int64_t*, const double*, const double*,
+                               int64_t, uint32_t, hipStream_t);
+
+// Prefix length of a query run: the smallest P with at most 128 lanes whose
+// lane state fits `budget` values, or 0.
+constexpr int query_p(int W, int D, int budget)
+{
+    for (int p = 1; p <= D; ++p) {
+        if (ipow(W, p) > 128) return 0;
+        int sub = 0;
+        for (int j = p; j <= D; ++j) sub += int(ipow(W, j - p));
+        if (sub + p <= budget) return p;
+    }
+    return 0;
+}
 
 template <typename T, int W, int D>
 constexpr QueryLauncher query_entry()
 {
-    constexpr int P = chen::choose_p(W, D, sizeof(T) == 4 ? chen::kBudgetF32 : chen::kBudgetF64);
-    if constexpr (ipow(W, P) <= 64)
+    constexpr int P = query_p(W, D, sizeof(T) == 4 ? 80 : 40);
+    if constexpr (P > 0)
         return &launch_query_fast<T, W, D, P>;
     else
         return nullptr;
 }
 
-// Shapes whose prefix split fits one wave, float32 and float64.
+// Shapes with a query kernel of their own, float32 and float64.
 template <typename T>
 QueryLauncher find_query_fast(int w, int d)
 {
@@ -412,7 +487,7 @@ QueryLauncher find_query_fast(int w, int d)
     if (w == W && d == D) return query_entry<T, W, D>();
     RPY_Q(2, 2) RPY_Q(2, 3) RPY_Q(2, 4) RPY_Q(2, 5) RPY_Q(2, 6) RPY_Q(2, 7) RPY_Q(2, 8)
     RPY_Q(3, 2) RPY_Q(3, 3) RPY_Q(3, 4) RPY_Q(3, 5) RPY_Q(3, 6)
-    RPY_Q(4, 2) RPY_Q(4, 3) RPY_Q(4, 4) RPY_Q(4, 5) RPY_Q(4, 6)
+    RPY_Q(4, 2) RPY_Q(4, 3) RPY_Q(4, 4) RPY_Q(4, 5)
     RPY_Q(5, 2) RPY_Q(5, 3) RPY_Q(5, 4) RPY_Q(5, 5)
     RPY_Q(6, 2) RPY_Q(6, 3) RPY_Q(6, 4)
     RPY_Q(7, 2) RPY_Q(7, 3) RPY_Q(7, 4)
@@ -526,8 +601,8 @@ int dyadic_cache_query(void* out, const void* cache, const void* increments, con
     const uint32_t grid = uint32_t(std::min<int64_t>(n_queries, 16384));
     const QueryLauncher fast = dtype == RPY_F64 ? find_query_fast<double>(width, depth)
                                                 : find_query_fast<float>(width, depth);
-    if (fast && fast(out, cache, increments, params, offsets, n_streams, b, pw, resolution,
-                     levels, q_stream, q_inf, q_sup, n_queries, grid, st)) {
+    if (fast && fast(out, cache, increments, params, offsets, n_streams, resolution, levels,
+                     q_stream, q_inf, q_sup, n_queries, grid, st)) {
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
     }

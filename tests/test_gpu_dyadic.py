@@ -58,6 +58,30 @@ def test_cache_nodes_and_queries(gpu, rng, dtype, tol):
         assert O.level_errors(out[q], ref, w, d).max() <= tol, (q, s, a[q], b[q])
 
 
+@pytest.mark.parametrize("w,d,dtype,tol", [(5, 5, np.float64, 1e-12), (8, 4, np.float32, 2e-5),
+                                           (2, 7, np.float32, 2e-5)])
+def test_queries_of_long_runs_and_many_nodes(gpu, rng, w, d, dtype, tol):
+    """Streams longer than one staging chunk per fine run (resolution 10,
+    stored levels 2, so a fine run spans up to 256 leaves) and queries over
+    many stored nodes, against the oracle signature of the selected rows."""
+    r, J = 10, 2
+    inc, params, offs, lens = _streams(rng, 4, w, 600, 1100, dtype)
+    cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J, dtype=dtype)
+    nq = 16
+    sid = rng.integers(0, 4, size=nq)
+    a = np.floor(rng.uniform(0, 1, size=nq) * 2 ** r) / 2 ** r
+    b = np.floor(rng.uniform(0, 1, size=nq) * 2 ** r) / 2 ** r
+    a, b = np.minimum(a, b), np.maximum(a, b)
+    a[0], b[0] = 1 / 2 ** r, 1 - 1 / 2 ** r
+    out = cache.query(sid, a, b).cpu().numpy()
+    inc64 = inc.astype(np.float64)
+    for q in range(nq):
+        s = sid[q]
+        sel = (params[offs[s]:offs[s + 1]] >= a[q]) & (params[offs[s]:offs[s + 1]] < b[q])
+        ref = O.signature(inc64[offs[s]:offs[s + 1]][sel], w, d)
+        assert O.level_errors(out[q], ref, w, d).max() <= tol, q
+
+
 def test_cache_benchmark_shape_sample(gpu):
     """Width 5, depth 5, resolution 12, stored levels 6 -- the benchmark's
     cache shape -- on 64 streams, with queries checked against the oracle."""
@@ -80,9 +104,9 @@ def test_cache_benchmark_shape_sample(gpu):
 
 
 def test_queries_on_the_workgroup_path(gpu, rng):
-    """Width 5, depth 5 in float64 has no one-wave prefix split (125 lanes),
-    so its queries take the all-workgroup kernel; same oracle check."""
-    w, d, r, J = 5, 5, 8, 3
+    """Width 2, depth 9 has no query kernel of its own, so its queries take
+    the all-workgroup kernel; same oracle check."""
+    w, d, r, J = 2, 9, 8, 3
     inc, params, offs, lens = _streams(rng, 3, w, 20, 200, np.float64)
     cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J,
                                     dtype=np.float64)
