@@ -71,6 +71,32 @@ def time_kernel(sig_fn, steps, warmup, torch):
     return start.elapsed_time(stop) / steps
 
 
+def cover_counts(A, B, r, J):
+    """Per query: the number of pieces of the maximal dyadic cover of [A, B)
+    (units of 2^-r) that are stored nodes (level <= J), vectorised over the
+    queries (the same left-to-right greedy cover as the query kernel)."""
+    A = A.astype(np.int64).copy()
+    B = B.astype(np.int64)
+    nodes = np.zeros(A.shape, dtype=np.int64)
+    active = A < B
+    while active.any():
+        a = A[active]
+        b = B[active]
+        lg = np.full(a.shape, r, dtype=np.int64)
+        nz = a != 0
+        low = a[nz] & -a[nz]
+        lg[nz] = np.log2(low).astype(np.int64)
+        while True:
+            over = a + (np.int64(1) << lg) > b
+            if not over.any():
+                break
+            lg[over] -= 1
+        nodes[np.flatnonzero(active)] += (r - lg) <= J
+        A[active] = a + (np.int64(1) << lg)
+        active = A < B
+    return nodes
+
+
 def dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs, n_queries=100000, levels=6,
                      resolution=12):
     """Dyadic cache of the same batch at resolution 12 (stored levels 0..6)
@@ -103,10 +129,20 @@ def dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs, n_queries=100000, le
     cache.build()
     torch.cuda.synchronize()
     rebuild_ms = (time.perf_counter() - t0) * 1e3
+    # algorithmic HBM bytes of the query batch: every stored node a query
+    # multiplies is read once, every answer written once (raw fine-run rows
+    # are < 1% and left out)
+    tsize = int(cache.tsize)
+    nodes = cover_counts(np.minimum(a, b) * 2 ** resolution, np.maximum(a, b) * 2 ** resolution,
+                         resolution, levels)
+    q_bytes = float((nodes.sum() + n_queries) * tsize * 4)
     res = {"levels_stored": levels, "resolution": resolution, "dtype": "f32",
            "cache_bytes": int(cache.cache.numel()) * 4, "build_ms": rebuild_ms,
            "first_build_ms": build_ms, "queries": n_queries, "query_ms": q_ms,
-           "queries_per_s": n_queries / (q_ms * 1e-3)}
+           "queries_per_s": n_queries / (q_ms * 1e-3),
+           "query_nodes_mean": float(nodes.mean()),
+           "query_hbm_gb_s": q_bytes / (q_ms * 1e-3) / 1e9,
+           "query_hbm_frac": q_bytes / (q_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
     del cache, out
     torch.cuda.empty_cache()
     return res
@@ -362,10 +398,11 @@ def main():
     L = _capi.lib()
 
     def make_fn(dtype):
-        out = torch.empty((N_STREAMS, tsize), dtype=torch.float32 if dtype == "f32" else torch.float64,
-                          device="cuda")
-        src = dinc32 if dtype == "f32" else dinc64
-        code = _capi.RPY_F32 if dtype == "f32" else _capi.RPY_F64
+        out = torch.empty((N_STREAMS, tsize),
+                          dtype=torch.float64 if dtype == "f64" else torch.float32, device="cuda")
+        src = dinc64 if dtype == "f64" else dinc32
+        code = {"f32": _capi.RPY_F32, "f64": _capi.RPY_F64,
+                "f32c": _capi.RPY_F32_COMPENSATED}[dtype]
 
         def fn():
             _capi.check(L.rpy_stream_signature_batch(out.data_ptr(), src.data_ptr(),
@@ -374,7 +411,7 @@ def main():
         return fn, out
 
     results = {}
-    for dt in ("f32", "f64"):
+    for dt in ("f32", "f64", "f32c"):
         fn, out = make_fn(dt)
         ms = time_kernel(fn, args.steps if dt == args.dtype else max(2, args.steps // 2),
                          args.warmup, torch)
@@ -422,11 +459,16 @@ def main():
                        "value": N_STREAMS / (results["f32"][0] * 1e-3),
                        "tflops": flops / (results["f32"][0] * 1e-3) / 1e12},
         "rel_err_f32_vs_f64": err,
+        "f32_compensated_kernel": {
+            "ms_per_step": results["f32c"][0],
+            "value": N_STREAMS / (results["f32c"][0] * 1e-3),
+            "arithmetic": "float64 on float32 data, rounded once",
+            "rel_err_vs_f64": level_rel_err(results["f32c"][1], s64, W, D)},
     }
     legs = [x for x in args.legs.split(",") if x]
     if "dyadic" in legs:
         line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
-    del dinc64, results, s32, s64
+    del dinc64, results, s32, s64, out
     torch.cuda.empty_cache()
     if "fma" in legs:
         line["fma_w4_d6"] = fma_leg(torch, max(3, args.steps // 2), 1)

@@ -207,7 +207,8 @@ int rpy_dense_sparse_map(void *out, const void *arg,
  * (algebra/src/lite_context.h:545-563) applied per stream, the value
  * LieIncrementStream.signature() returns for the stream's full support
  * (streams/src/stream.cpp:156-171).  dtype: RPY_F64 (double data),
- * RPY_F32 or RPY_F32_COMPENSATED (float data).  Work is one persistent
+ * RPY_F32 or RPY_F32_COMPENSATED (float data, float64 arithmetic, each
+ * coefficient rounded to float once at the end).  Work is one persistent
  * launch over a length-sorted queue. */
 int rpy_stream_signature_batch(void *out, const void *increments,
                                const int64_t *offsets, int64_t n_streams,

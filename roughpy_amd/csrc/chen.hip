@@ -15,6 +15,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "chen.hpp"
@@ -58,10 +59,12 @@ struct Staging {
     static constexpr int kPerLane = (kVals + kWave - 1) / kWave;
 };
 
-template <typename T, int W, int D, int P>
+// T: arithmetic type; IO: type of the increments and signatures in memory
+// (float data with double arithmetic is the compensated float32 mode).
+template <typename T, int W, int D, int P, typename IO = T>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2)))
 void chen_persistent_kernel(
-        T* __restrict__ out, const T* __restrict__ incr, const int64_t* __restrict__ begins,
+        IO* __restrict__ out, const IO* __restrict__ incr, const int64_t* __restrict__ begins,
         const int64_t* __restrict__ ends, const int32_t* __restrict__ order, int64_t n_items,
         int64_t tsize, DevBasis b, uint32_t* __restrict__ counter)
 {
@@ -140,7 +143,7 @@ void chen_persistent_kernel(
                 const int r = rem / W;
                 T val = T(0);
                 if (v < St::kVals && sl < n_slots && t0 + r < slot_len[sl])
-                    val = incr[(slot_row0[sl] + t0) * W + rem];
+                    val = T(incr[(slot_row0[sl] + t0) * W + rem]);
                 pre[i] = val;
             }
         };
@@ -177,19 +180,19 @@ void chen_persistent_kernel(
 
         const int s = slot_stream[slot];
         if (active) {
-            T* o = out + int64_t(s) * tsize;
+            IO* o = out + int64_t(s) * tsize;
             // subtree levels: lane q owns words q*W^(k-P) .. (q+1)*W^(k-P)-1
             chen::static_for<D - P + 1>([&](auto i) {
                 constexpr int k = P + int(decltype(i)::value);
                 constexpr int nk = int(ipow(W, k - P));
-                T* ok = o + b.db[k] + int64_t(q) * nk;
-                chen::static_for<nk>([&](auto v) { ok[v] = st.sub[Sh::off(k) + v]; });
+                IO* ok = o + b.db[k] + int64_t(q) * nk;
+                chen::static_for<nk>([&](auto v) { ok[v] = IO(st.sub[Sh::off(k) + v]); });
             });
             // path levels: written by the first lane of each prefix group
             chen::static_for<P>([&](auto i) {
                 constexpr int k = int(decltype(i)::value);
                 constexpr int grp = int(ipow(W, P - k));
-                if (q % grp == 0) o[b.db[k] + q / grp] = st.path[k];
+                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.path[k]);
             });
         }
     }
@@ -198,21 +201,27 @@ void chen_persistent_kernel(
 // Fallback for shapes without a specialised kernel: one workgroup per
 // stream, the running signature in the output buffer, per-coefficient Horner
 // with a barrier between levels.
-template <typename T>
-__global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
-                                                          const T* __restrict__ incr,
+// With IO != T the running signature lives in a per-workgroup slot of
+// `work` and is rounded into `out` at the end of each stream.
+template <typename T, typename IO = T>
+__global__ __launch_bounds__(256) void chen_generic_kernel(IO* __restrict__ out,
+                                                          const IO* __restrict__ incr,
                                                           const int64_t* __restrict__ begins,
                                                           const int64_t* __restrict__ ends,
                                                           int64_t n_streams, int64_t tsize,
-                                                          DevBasis b)
+                                                          DevBasis b, T* __restrict__ work)
 {
     const int W = b.width, D = b.depth;
     for (int64_t s = blockIdx.x; s < n_streams; s += gridDim.x) {
-        T* S = out + s * tsize;
+        T* S;
+        if constexpr (std::is_same<T, IO>::value)
+            S = out + s * tsize;
+        else
+            S = work + int64_t(blockIdx.x) * tsize;
         for (int64_t i = threadIdx.x; i < tsize; i += blockDim.x) S[i] = (i == 0) ? T(1) : T(0);
         __syncthreads();
         for (int64_t r = begins[s]; r < ends[s]; ++r) {
-            const T* x = incr + r * W;
+            const IO* x = incr + r * W;
             for (int m = D; m >= 1; --m) {
                 const int64_t n = b.level_size(m);
                 for (int64_t idx = threadIdx.x; idx < n; idx += blockDim.x) {
@@ -223,7 +232,7 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
                         const int64_t pre = idx / scale;  // prefix of length K
                         const int letter = int(rest / scale);
                         rest -= int64_t(letter) * scale;
-                        p = p * (x[letter] / T(m - K + 1)) + S[b.db[K] + pre];
+                        p = p * (T(x[letter]) / T(m - K + 1)) + S[b.db[K] + pre];
                         scale = (scale >= W) ? scale / W : 1;
                     }
                     // only this thread reads S_m[idx]; lower levels stay old
@@ -233,6 +242,10 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(T* __restrict__ out,
                 __syncthreads();
             }
         }
+        if constexpr (!std::is_same<T, IO>::value) {
+            for (int64_t i = threadIdx.x; i < tsize; i += blockDim.x) out[s * tsize + i] = IO(S[i]);
+            __syncthreads();
+        }
     }
 }
 
@@ -240,7 +253,7 @@ using Launcher = int (*)(void* out, const void* incr, const int64_t* begins, con
                          const int32_t* order, int64_t n_streams, const DevBasis& b,
                          uint32_t* counter, hipStream_t st);
 
-template <typename T, int W, int D, int P>
+template <typename T, int W, int D, int P, typename IO = T>
 int launch_persistent(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
                       const int32_t* order,
                       int64_t n_streams, const DevBasis& b, uint32_t* counter, hipStream_t st)
@@ -251,12 +264,12 @@ int launch_persistent(void* out, const void* incr, const int64_t* begins, const 
     RPY_HIP_CHECK(hipGetDevice(&dev));
     RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, chen_persistent_kernel<T, W, D, P>, kWave, 0));
+            &per_cu, chen_persistent_kernel<T, W, D, P, IO>, kWave, 0));
     if (per_cu < 1) per_cu = 1;
     const int64_t chunks = (n_items + kWave - 1) / kWave;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(chunks, int64_t(cus) * per_cu));
-    hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P>), dim3(uint32_t(grid)), dim3(kWave), 0,
-                       st, (T*)out, (const T*)incr, begins, ends, order, n_items, b.size(), b,
+    hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P, IO>), dim3(uint32_t(grid)), dim3(kWave),
+                       0, st, (IO*)out, (const IO*)incr, begins, ends, order, n_items, b.size(), b,
                        counter);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
@@ -268,7 +281,7 @@ using chen::kBudgetF64;
 
 struct Entry {
     int w, d, p32, p64;
-    Launcher f32, f64;
+    Launcher f32, f64, f32c;
 };
 
 template <int W, int D>
@@ -277,7 +290,8 @@ Entry make_entry()
     constexpr int P32 = choose_p(W, D, kBudgetF32);
     constexpr int P64 = choose_p(W, D, kBudgetF64);
     return Entry{W, D, P32, P64, &launch_persistent<float, W, D, P32>,
-                 &launch_persistent<double, W, D, P64>};
+                 &launch_persistent<double, W, D, P64>,
+                 &launch_persistent<double, W, D, P64, float>};
 }
 
 const std::vector<Entry>& table()
@@ -317,9 +331,7 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
                             const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
                             rpy_dtype dtype, void* stream)
 {
-    if (dtype == RPY_F32_COMPENSATED)
-        return fail(RPY_ERR_UNSUPPORTED, "compensated float32 signatures are not available yet");
-    if (dtype != RPY_F32 && dtype != RPY_F64)
+    if (dtype != RPY_F32 && dtype != RPY_F64 && dtype != RPY_F32_COMPENSATED)
         return fail(RPY_ERR_DTYPE, "unsupported dtype");
     rpy_tensor_basis tb{width, depth, nullptr};
     DevBasis b;
@@ -336,14 +348,27 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
     const Entry* e = find_entry(width, depth);
     if (!e) {
         const int grid = int(std::min<int64_t>(n_streams, 4096));
-        if (dtype == RPY_F64)
-            hipLaunchKernelGGL(chen_generic_kernel<double>, dim3(grid), dim3(256), 0, st,
+        if (dtype == RPY_F64) {
+            hipLaunchKernelGGL((chen_generic_kernel<double>), dim3(grid), dim3(256), 0, st,
                                (double*)out, (const double*)increments, begins, ends, n_streams,
-                               b.size(), b);
-        else
-            hipLaunchKernelGGL(chen_generic_kernel<float>, dim3(grid), dim3(256), 0, st,
+                               b.size(), b, (double*)nullptr);
+        } else if (dtype == RPY_F32) {
+            hipLaunchKernelGGL((chen_generic_kernel<float>), dim3(grid), dim3(256), 0, st,
                                (float*)out, (const float*)increments, begins, ends, n_streams,
-                               b.size(), b);
+                               b.size(), b, (float*)nullptr);
+        } else {
+            double* work = nullptr;
+            RPY_HIP_CHECK(hipMallocAsync((void**)&work, size_t(grid) * size_t(b.size()) * 8, st));
+            hipLaunchKernelGGL((chen_generic_kernel<double, float>), dim3(grid), dim3(256), 0, st,
+                               (float*)out, (const float*)increments, begins, ends, n_streams,
+                               b.size(), b, work);
+            hipError_t err = hipGetLastError();
+            hipError_t ferr = hipFreeAsync(work, st);
+            if (err != hipSuccess)
+                return fail(RPY_ERR_HIP, "signature launch failed: %s", hipGetErrorString(err));
+            RPY_HIP_CHECK(ferr);
+            return RPY_OK;
+        }
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
     }
@@ -379,7 +404,7 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
         if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
     }
     if (!rc) {
-        Launcher f = (dtype == RPY_F64) ? e->f64 : e->f32;
+        Launcher f = (dtype == RPY_F64) ? e->f64 : (dtype == RPY_F32) ? e->f32 : e->f32c;
         rc = f(out, increments, begins, ends, vals_out, n_streams, b, counter, st);
     }
     hipError_t ferr = hipFreeAsync(ws, st);

@@ -181,3 +181,18 @@ def test_benchmark_size_properties(gpu):
     prod = rc.ft_mul(rc.FreeTensor(halves[0::2].contiguous(), tb),
                      rc.FreeTensor(halves[1::2].contiguous(), tb))
     assert O.level_errors(prod.data.cpu().numpy(), a64, w, d).max() <= 1e-12
+
+
+@pytest.mark.parametrize("w,d", [(5, 5), (3, 4), (9, 3)])
+def test_compensated_signatures_round_once(gpu, rng, w, d):
+    """Compensated float32 signatures (float data, float64 arithmetic) equal
+    the float64 oracle on the same float32 increments rounded to float once:
+    normwise level error within 2^-23.  (9, 3) runs on the generic kernel."""
+    lens = rng.integers(1, 300, size=9)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / 8).astype(np.float32)
+    got = rs.signature_batch(inc, offs, w, d, compensated=True)
+    assert got.dtype == np.float32
+    for s in range(len(lens)):
+        ref = O.signature(inc[offs[s]:offs[s + 1]].astype(np.float64), w, d)
+        assert O.level_errors(got[s].astype(np.float64), ref, w, d).max() <= 2.0 ** -23
