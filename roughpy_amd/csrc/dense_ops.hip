@@ -23,6 +23,7 @@
 
 #include <algorithm>
 
+#include "chen_core.hpp"
 #include "dense_ops.hpp"
 #include "kernel_util.hpp"
 #include "rpy_common.hpp"
@@ -392,11 +393,156 @@ int check_batch(int64_t batch, std::initializer_list<int64_t> strides, int64_t n
     return RPY_OK;
 }
 
+// ---------------------------------------------------------------------------
+// out += lhs (x) rhs at full depth for a compile-time shape: level offsets and
+// divisors are constants, every term is unrolled and all loads of an output
+// batch are issued before its arithmetic.  Levels below D are staged in LDS
+// already converted to the accumulator type; the top level of lhs and rhs
+// and every level of out are read once from global memory.  Same summation
+// order as ft_fma_kernel (free_tensor_fma.hpp:13-53).
+// ---------------------------------------------------------------------------
+template <int W>
+constexpr int fdb(int k)
+{
+    int s = 0;
+    for (int j = 0; j < k; ++j) s += int(ipow(W, j));
+    return s;
+}
+
+// One output of level m: lhs degree m (rhs unit), then m-1 .. 1, then 0 (lhs unit).
+template <typename Acc, int W, int m>
+RPY_FI Acc fma_output(Acc acc, Acc top_l, Acc top_r, int idx, const Acc* sl, const Acc* sr)
+{
+    Acc a = acc + top_l * sr[0];
+    Acc lv[m > 1 ? m - 1 : 1], rv[m > 1 ? m - 1 : 1];
+    chen::static_for<m - 1>([&](auto k0) {
+        constexpr int ld = m - 1 - int(decltype(k0)::value);
+        constexpr int rd = m - ld;
+        constexpr int wr = int(ipow(W, rd));
+        const int i = idx / wr;
+        lv[k0] = sl[fdb<W>(ld) + i];
+        rv[k0] = sr[fdb<W>(rd) + (idx - i * wr)];
+    });
+    chen::static_for<m - 1>([&](auto k0) { a += lv[k0] * rv[k0]; });
+    return a + sl[0] * top_r;
+}
+
+template <typename T, typename Acc, int W, int D>
+__global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* __restrict__ out,
+                                                             const T* __restrict__ lhs,
+                                                             const T* __restrict__ rhs,
+                                                             int64_t batch, int64_t os,
+                                                             int64_t ls, int64_t rs)
+{
+    constexpr int kStaged = fdb<W>(D);
+    constexpr int kN = int(ipow(W, D));          // top-level outputs
+    constexpr int kTop = (kN + kBlock - 1) / kBlock;
+    constexpr int kTopU = kTop < 16 ? kTop : 16;  // top outputs held per thread per pass
+    constexpr int dbD = fdb<W>(D);
+    __shared__ Acc sl[kStaged];
+    __shared__ Acc sr[kStaged];
+    constexpr int kU = 4;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        T* o = out + e * os;
+        const T* l = lhs + e * ls;
+        const T* r = rhs + e * rs;
+        // the first top-level pass's loads go out before the staging loads
+        T t_o[kTopU], t_l[kTopU], t_r[kTopU];
+        auto load_top = [&](int base) {
+#pragma unroll
+            for (int q = 0; q < kTopU; ++q) {
+                const int idx = base + q * kBlock;
+                const bool ok = idx < kN;
+                t_o[q] = ok ? o[dbD + idx] : T(0);
+                t_l[q] = ok ? l[dbD + idx] : T(0);
+                t_r[q] = ok ? r[dbD + idx] : T(0);
+            }
+        };
+        load_top(threadIdx.x);
+        for (int i = threadIdx.x; i < kStaged; i += kBlock) {
+            sl[i] = Acc(l[i]);
+            sr[i] = Acc(r[i]);
+        }
+        __syncthreads();
+        for (int base = threadIdx.x; base < kN; base += kTopU * kBlock) {
+            if (base != int(threadIdx.x)) load_top(base);
+            // the operands are in registers already; two outputs at a time keep
+            // the LDS loads of a pair in flight without inflating the VGPRs
+#pragma unroll 2
+            for (int q = 0; q < kTopU; ++q) {
+                const int idx = base + q * kBlock;
+                if (idx < kN)
+                    o[dbD + idx] = T(fma_output<Acc, W, D>(Acc(t_o[q]), Acc(t_l[q]), Acc(t_r[q]),
+                                                            idx, sl, sr));
+            }
+        }
+        chen::static_for<D - 1>([&](auto m0) {
+            constexpr int m = D - 1 - int(decltype(m0)::value);
+            constexpr int n = int(ipow(W, m));
+            constexpr int dbm = fdb<W>(m);
+            for (int base = threadIdx.x; base < n; base += kU * kBlock) {
+                Acc acc[kU];
+#pragma unroll
+                for (int q = 0; q < kU; ++q) {
+                    const int idx = base + q * kBlock;
+                    acc[q] = idx < n ? Acc(o[dbm + idx]) : Acc(0);
+                }
+#pragma unroll
+                for (int q = 0; q < kU; ++q) {
+                    const int idx = base + q * kBlock;
+                    if (idx < n)
+                        o[dbm + idx] = T(fma_output<Acc, W, m>(acc[q], sl[dbm + idx],
+                                                               sr[dbm + idx], idx, sl, sr));
+                }
+            }
+        });
+        if (threadIdx.x == 0) o[0] = T(Acc(o[0]) + Acc(l[0]) * Acc(r[0]));
+        __syncthreads();
+    }
+}
+
+template <typename T, typename Acc, int W, int D>
+void launch_fma_fixed(void* out, const void* lhs, const void* rhs, int64_t batch, int64_t os,
+                      int64_t ls, int64_t rs, hipStream_t st)
+{
+    hipLaunchKernelGGL((ft_fma_fixed_kernel<T, Acc, W, D>), dim3(grid_for(batch)),
+                       dim3(kBlock), 0, st, (T*)out, (const T*)lhs, (const T*)rhs, batch, os, ls,
+                       rs);
+}
+
+using FmaFixed = void (*)(void*, const void*, const void*, int64_t, int64_t, int64_t, int64_t,
+                          hipStream_t);
+
+template <typename T, typename Acc>
+FmaFixed find_fma_fixed(int w, int d)
+{
+#define RPY_F(W, D) \
+    if (w == W && d == D) return &launch_fma_fixed<T, Acc, W, D>;
+    RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
+    RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
+    RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
+    RPY_F(5, 2) RPY_F(5, 3) RPY_F(5, 4) RPY_F(5, 5)
+    RPY_F(6, 2) RPY_F(6, 3) RPY_F(6, 4) RPY_F(6, 5)
+    RPY_F(7, 2) RPY_F(7, 3) RPY_F(7, 4)
+    RPY_F(8, 2) RPY_F(8, 3) RPY_F(8, 4)
+#undef RPY_F
+    return nullptr;
+}
+
 template <typename T, typename Acc = T>
 int launch_fma(void* out, const void* lhs, const void* rhs, const DevBasis& b, Bounds ob,
                Bounds lb, Bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
                hipStream_t st)
 {
+    const bool full = ob.lo == 0 && lb.lo == 0 && rb.lo == 0 && ob.hi == b.depth &&
+                      lb.hi == b.depth && rb.hi == b.depth;
+    if (full) {
+        if (FmaFixed f = find_fma_fixed<T, Acc>(b.width, b.depth)) {
+            f(out, lhs, rhs, batch, os, ls, rs, st);
+            RPY_HIP_CHECK(hipGetLastError());
+            return RPY_OK;
+        }
+    }
     const Pow pw = make_pow(b.width, b.depth);
     const int top = ob.hi;
     const size_t lds = size_t(2) * size_t(b.db[top]) * sizeof(T);
