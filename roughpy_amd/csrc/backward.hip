@@ -443,7 +443,7 @@ int stream_signature_vjp(void* grad, const void* increments, const int64_t* offs
     // the kernel's static LDS (increment, reciprocals, reduction) is < 4 KiB
     const bool in_lds = lds + 4096 <= lds_limit(st);
     if (!in_lds)
-        RPY_HIP_CHECK(hipMallocAsync(&scratch, size_t(grid) * 4 * size_t(b.size()) * es, st));
+        RPY_HIP_CHECK(workspace_alloc(&scratch, size_t(grid) * 4 * size_t(b.size()) * es, st));
     const dim3 g = dim3(uint32_t(grid)), blk = dim3(kBlock);
     const size_t sh = in_lds ? lds : 0;
     if (in_lds) {

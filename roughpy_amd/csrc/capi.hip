@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -53,6 +55,24 @@ int fail(int code, const char* fmt, ...)
     va_end(ap);
     g_last_error = buf;
     return code;
+}
+
+hipError_t workspace_alloc(void** ptr, size_t bytes, hipStream_t stream)
+{
+    constexpr int kMaxDevices = 64;
+    static std::atomic<bool> kept[kMaxDevices];
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev >= 0 && dev < kMaxDevices && !kept[dev].load(std::memory_order_acquire)) {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t threshold = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
+        kept[dev].store(true, std::memory_order_release);
+    }
+    return hipMallocAsync(ptr, bytes, stream);
 }
 
 int make_basis(const rpy_tensor_basis* tb, DevBasis& out)
@@ -238,7 +258,7 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
     // sig (dtype) | sig widened to f64 (if needed) | log (f64)
     char* ws = nullptr;
     const size_t bytes = tel * elt + (dtype == RPY_F64 ? 0 : tel * 8) + tel * 8;
-    RPY_HIP_CHECK(hipMallocAsync((void**)&ws, bytes, st));
+    RPY_HIP_CHECK(workspace_alloc((void**)&ws, bytes, st));
     void* sig = ws;
     double* sig64 = (dtype == RPY_F64) ? (double*)ws : (double*)(ws + tel * elt);
     double* lg = (double*)(ws + tel * elt + (dtype == RPY_F64 ? 0 : tel * 8));

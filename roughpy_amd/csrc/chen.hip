@@ -358,7 +358,7 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
                                b.size(), b, (float*)nullptr);
         } else {
             double* work = nullptr;
-            RPY_HIP_CHECK(hipMallocAsync((void**)&work, size_t(grid) * size_t(b.size()) * 8, st));
+            RPY_HIP_CHECK(workspace_alloc((void**)&work, size_t(grid) * size_t(b.size()) * 8, st));
             hipLaunchKernelGGL((chen_generic_kernel<double, float>), dim3(grid), dim3(256), 0, st,
                                (float*)out, (const float*)increments, begins, ends, n_streams,
                                b.size(), b, work);
@@ -384,7 +384,7 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
     const size_t v_bytes = align(n * sizeof(int32_t));
     const size_t total = 2 * k_bytes + 2 * v_bytes + 256 + align(sort_bytes);
     char* ws = nullptr;
-    RPY_HIP_CHECK(hipMallocAsync((void**)&ws, total, st));
+    RPY_HIP_CHECK(workspace_alloc((void**)&ws, total, st));
     uint32_t* keys_in = (uint32_t*)ws;
     uint32_t* keys_out = (uint32_t*)(ws + k_bytes);
     int32_t* vals_in = (int32_t*)(ws + 2 * k_bytes);

@@ -1,11 +1,19 @@
 // chen.hpp -- entry points of chen.hip.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "rpy_common.hpp"
 
 namespace rpy_amd {
+
+// Stream-ordered temporary allocation from the device's default memory
+// pool.  On first use per device the pool's release threshold is raised so
+// freed blocks stay cached and later calls do not go back to the driver
+// (rpy_release_workspace trims the pool).
+hipError_t workspace_alloc(void** ptr, size_t bytes, hipStream_t stream);
 
 int stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
                            int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,

@@ -545,7 +545,7 @@ int dyadic_cache_build(void* cache, const void* increments, const double* params
     const size_t es = dtype == RPY_F64 ? 8 : 4;
 
     int64_t* bounds = nullptr;
-    RPY_HIP_CHECK(hipMallocAsync((void**)&bounds, size_t(2 * nseg) * sizeof(int64_t), st));
+    RPY_HIP_CHECK(workspace_alloc((void**)&bounds, size_t(2 * nseg) * sizeof(int64_t), st));
     hipLaunchKernelGGL(segment_bounds_kernel, dim3(blocks_for(nseg)), dim3(kBlock), 0, st, params,
                        offsets, n_streams, J, bounds, bounds + nseg);
     hipError_t err = hipGetLastError();
