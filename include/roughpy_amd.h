@@ -340,13 +340,14 @@ int rpy_debug_signature_host_emulation(double *out, const double *increments,
                                        int64_t length, int32_t width,
                                        int32_t depth, int32_t prefix);
 
-/* Runs the lane algebra of the one-wave-per-stream signature backward on the
- * host for one float64 stream (host arrays): grad (length x width) from the
- * increments, the stream's signature and the cotangent of the signature.
- * Shapes: width 8 depth 2..4, width 4 depth 3..6, width 2 depth 6..7. */
+/* Runs the lane algebra of the register-resident signature backward on the
+ * host for one float64 stream (host arrays) with prefix length `prefix`:
+ * grad (length x width) from the increments, the stream's signature and the
+ * cotangent of the signature.  (width, depth, prefix): (8, 2..3, 2),
+ * (8, 4, 2..3), (4, 3..5, 3), (4, 6, 3..4), (2, 6, 6), (2, 7, 6..7). */
 int rpy_debug_vjp_host_emulation(double *grad, const double *increments,
                                  int64_t length, int32_t width, int32_t depth,
-                                 const double *signature,
+                                 int32_t prefix, const double *signature,
                                  const double *grad_signature);
 
 /* Prefix length the signature kernel uses for (width, depth, dtype), or 0 when

@@ -16,11 +16,11 @@
 //   dl/dx_a += sum_{w'} adjP_K(w'a) P_{K-1}(w') / (M-K+1),
 //   H_{t-1,K} = sum_{M >= K} adjP^{(M)}_K   (= R*_{exp(x_t)} H_t).
 // Two kernels:
-// * sig_vjp_wave_kernel: one wave64 per stream, the 64 lanes splitting the
-//   words by prefix (chen_bwd.hpp), A and H in registers, cross-lane sums by
-//   shuffles; for the shapes whose prefix split has exactly 64 lanes and
-//   whose state fits the register file (width 8 depth <= 4, width 4 depth
-//   <= 6, width 2 depth 6..10 in float32).
+// * sig_vjp_wave_kernel: one workgroup of W^P lanes (whole waves) per
+//   stream, the lanes splitting the words by prefix (chen_bwd.hpp), A and H
+//   in registers, cross-lane sums by shuffles and, across waves, through LDS;
+//   widths 2, 4 and 8 (width 8 depth <= 4, width 4 depth <= 6, width 2
+//   depth 6..10).
 // * sig_vjp_kernel: any shape; one workgroup per stream with its four
 //   working tensors in LDS when they fit, else in a global scratch slot.
 //
@@ -234,25 +234,34 @@ RPY_HD inline __attribute__((always_inline)) void load_share(T* path, T* sub, co
     });
 }
 
+// One workgroup of NP = W^P threads per stream (one or more whole waves);
+// every path-level lane group (W^(P-k) lanes, k >= 1) lies inside one wave,
+// so only the W gradient totals cross waves (through LDS, once per step).
 template <typename T, int W, int D, int P>
-__global__ __launch_bounds__(64) void sig_vjp_wave_kernel(
+__global__ __launch_bounds__((chen::Shape<W, D, P>::NP)) void sig_vjp_wave_kernel(
         T* __restrict__ grad, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
         const T* __restrict__ sig, const T* __restrict__ gsig, DevBasis b)
 {
     using Sh = chen::Shape<W, D, P>;
-    static_assert(Sh::NP == 64, "one stream per wave64");
+    constexpr int kWaves = Sh::NP / 64;
+    static_assert(Sh::NP % 64 == 0 && kWaves <= 16, "whole waves per stream");
+    static_assert(ipow(W, P - 1) <= 64, "path groups inside one wave");
     constexpr int LW = (W == 2) ? 1 : (W == 4) ? 2 : 3;
     static_assert((1 << LW) == W, "width must be 2, 4 or 8");
+    __shared__ T red[2][kWaves > 1 ? kWaves : 1][W];
     const int64_t s = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int q_lane = threadIdx.x;          // prefix index
+    const int lane = threadIdx.x % 64;       // lane in the wave
+    const int wave = threadIdx.x / 64;
     const int64_t tsize = b.size();
     int u[P];
-    prefix_digits<W, P>(lane, u);
+    prefix_digits<W, P>(q_lane, u);
     chen::BwdLane<T, W, D, P> st;
     st.set_prefix(u);
-    load_share<T, W, D, P>(st.A.path, st.A.sub, sig + s * tsize, b, lane);
-    load_share<T, W, D, P>(st.hpath, st.hsub, gsig + s * tsize, b, lane);
+    load_share<T, W, D, P>(st.A.path, st.A.sub, sig + s * tsize, b, q_lane);
+    load_share<T, W, D, P>(st.hpath, st.hsub, gsig + s * tsize, b, q_lane);
     const int64_t r0 = offsets[s], r1 = offsets[s + 1];
+    int parity = 0;
     for (int64_t row = r1 - 1; row >= r0; --row) {
         const T* xr = incr + row * W;
         T tab[D * W];
@@ -294,7 +303,20 @@ __global__ __launch_bounds__(64) void sig_vjp_wave_kernel(
         });
         T tot = v[0];
         for (int sft = 32 >> LW; sft >= 1; sft >>= 1) tot += __shfl_xor(tot, sft);
-        if ((lane & ((64 >> LW) - 1)) == 0) grad[row * W + (lane >> (6 - LW))] = tot;
+        const bool writer = (lane & ((64 >> LW) - 1)) == 0;
+        if constexpr (kWaves == 1) {
+            if (writer) grad[row * W + (lane >> (6 - LW))] = tot;
+        } else {
+            // wave totals to LDS (two buffers, so one barrier per step)
+            if (writer) red[parity][wave][lane >> (6 - LW)] = tot;
+            __syncthreads();
+            if (threadIdx.x < W) {
+                T g = T(0);
+                for (int w = 0; w < kWaves; ++w) g += red[parity][w][threadIdx.x];
+                grad[row * W + threadIdx.x] = g;
+            }
+            parity ^= 1;
+        }
     }
 }
 
@@ -353,7 +375,8 @@ template <typename T, int W, int D, int P>
 void launch_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n,
                  const void* sig, const void* gsig, const DevBasis& b, hipStream_t st)
 {
-    hipLaunchKernelGGL((sig_vjp_wave_kernel<T, W, D, P>), dim3(uint32_t(n)), dim3(64), 0, st,
+    hipLaunchKernelGGL((sig_vjp_wave_kernel<T, W, D, P>), dim3(uint32_t(n)),
+                       dim3(chen::Shape<W, D, P>::NP), 0, st,
                        (T*)grad, (const T*)incr, offsets, (const T*)sig, (const T*)gsig, b);
 }
 
@@ -373,6 +396,9 @@ bool try_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n, c
     RPY_WAVE(2, 6, 6) RPY_WAVE(2, 7, 6) RPY_WAVE(2, 8, 6) RPY_WAVE(2, 9, 6)
     if constexpr (std::is_same<T, float>::value) {
         RPY_WAVE(8, 4, 2) RPY_WAVE(4, 6, 3) RPY_WAVE(2, 10, 6)
+    } else {
+        // wider splits where the double state would not fit one wave's lanes
+        RPY_WAVE(8, 4, 3) RPY_WAVE(4, 6, 4) RPY_WAVE(2, 10, 7)
     }
 #undef RPY_WAVE
     return false;
@@ -390,20 +416,20 @@ size_t lds_limit(hipStream_t)
 }  // namespace
 
 int vjp_host_emulation(double* grad, const double* incr, int64_t len, int32_t width,
-                       int32_t depth, const double* sig, const double* gsig)
+                       int32_t depth, int32_t prefix, const double* sig, const double* gsig)
 {
     rpy_tensor_basis tb{width, depth, nullptr};
     DevBasis b;
     int rc = make_basis(&tb, b);
     if (rc) return rc;
 #define RPY_EMU(W, D, P)                                           \
-    if (width == W && depth == D) {                                \
+    if (width == W && depth == D && prefix == P) {                 \
         vjp_emulate<W, D, P>(grad, incr, len, sig, gsig, b);       \
         return RPY_OK;                                             \
     }
-    RPY_EMU(8, 2, 2) RPY_EMU(8, 3, 2) RPY_EMU(8, 4, 2)
-    RPY_EMU(4, 3, 3) RPY_EMU(4, 4, 3) RPY_EMU(4, 5, 3) RPY_EMU(4, 6, 3)
-    RPY_EMU(2, 6, 6) RPY_EMU(2, 7, 6)
+    RPY_EMU(8, 2, 2) RPY_EMU(8, 3, 2) RPY_EMU(8, 4, 2) RPY_EMU(8, 4, 3)
+    RPY_EMU(4, 3, 3) RPY_EMU(4, 4, 3) RPY_EMU(4, 5, 3) RPY_EMU(4, 6, 3) RPY_EMU(4, 6, 4)
+    RPY_EMU(2, 6, 6) RPY_EMU(2, 7, 6) RPY_EMU(2, 7, 7)
 #undef RPY_EMU
     return fail(RPY_ERR_UNSUPPORTED, "no host emulation of the wave backward for this shape");
 }

@@ -292,12 +292,12 @@ int rpy_debug_signature_host_emulation(double* out, const double* increments, in
 }
 
 int rpy_debug_vjp_host_emulation(double* grad, const double* increments, int64_t length,
-                                 int32_t width, int32_t depth, const double* signature,
-                                 const double* grad_signature)
+                                 int32_t width, int32_t depth, int32_t prefix,
+                                 const double* signature, const double* grad_signature)
 {
     if (!grad || !signature || !grad_signature || (!increments && length > 0))
         return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
-    return vjp_host_emulation(grad, increments, length, width, depth, signature,
+    return vjp_host_emulation(grad, increments, length, width, depth, prefix, signature,
                               grad_signature);
 }
 

@@ -50,7 +50,7 @@ int dyadic_cover_host(int64_t A, int64_t B, int r, int64_t* ks, int* ns, int max
 
 // backward.hip
 int vjp_host_emulation(double* grad, const double* incr, int64_t len, int32_t width,
-                       int32_t depth, const double* sig, const double* gsig);
+                       int32_t depth, int32_t prefix, const double* sig, const double* gsig);
 int stream_signature_vjp(void* grad, const void* increments, const int64_t* offsets,
                          int64_t n_streams, int32_t width, int32_t depth, const void* signatures,
                          const void* grad_signatures, rpy_dtype dtype, void* stream);

@@ -195,18 +195,20 @@ def test_kernel_prefix_choice():
     assert L.rpy_debug_signature_prefix(9, 3, 1) == 0      # generic kernel
 
 
-@pytest.mark.parametrize("w,d", [(8, 2), (8, 3), (8, 4), (4, 3), (4, 5), (4, 6), (2, 6), (2, 7)])
-def test_backward_lane_algebra_on_host(rng, w, d):
-    """Host run of the wave backward's lane algebra (64 prefix lanes, path
-    shares summed across lanes) against float64 central differences (step
-    1e-6) of the oracle signature: d<G, S>/dx for every increment."""
+@pytest.mark.parametrize("w,d,p", [(8, 2, 2), (8, 3, 2), (8, 4, 2), (8, 4, 3), (4, 3, 3),
+                                   (4, 5, 3), (4, 6, 3), (4, 6, 4), (2, 6, 6), (2, 7, 6),
+                                   (2, 7, 7)])
+def test_backward_lane_algebra_on_host(rng, w, d, p):
+    """Host run of the register backward's lane algebra (W^P prefix lanes,
+    path shares summed across lanes) against float64 central differences
+    (step 1e-6) of the oracle signature: d<G, S>/dx for every increment."""
     L = _capi.lib()
     n = 7
     inc = np.ascontiguousarray(rng.normal(size=(n, w)) / 3)
     G = rng.normal(size=O.tensor_size(w, d))
     S = O.signature(inc, w, d)
     grad = np.full((n, w), np.nan)
-    _capi.check(L.rpy_debug_vjp_host_emulation(_capi.dblp(grad), _capi.dblp(inc), n, w, d,
+    _capi.check(L.rpy_debug_vjp_host_emulation(_capi.dblp(grad), _capi.dblp(inc), n, w, d, p,
                                                 _capi.dblp(S), _capi.dblp(G)))
     h = 1e-6
     for t in range(n):
