@@ -74,9 +74,10 @@ void chen_persistent_kernel(
     constexpr int kSlots = St::kSlots;
     constexpr int kBlk = St::kBlk;
     constexpr int kBlkVals = kBlk * W;       // raw values of one slot's block
-    // Scaled rows (x_a / d, d = 1..D) unless the lane is wide: a float64 lane
-    // with P >= 3 would read more LDS bytes per step than the CU can serve.
-    constexpr bool kScaled = !(sizeof(T) == 8 && P >= 3);
+    // Scaled rows (x_a / d, d = 1..D) when a lane does at least 4 FMAs per
+    // table value it reads; below that the LDS bandwidth (128 B/clk/CU) is
+    // the limit and raw rows scaled in registers are cheaper.
+    constexpr bool kScaled = chen::lane_fmas<W, D, P>() >= 4 * Sh::ROW;
     constexpr int kRow = kScaled ? Sh::ROW : W;
 
     // stage[(slot * kBlk + r) * kRow + (d-1) * W + a] = x_a(row r) / d

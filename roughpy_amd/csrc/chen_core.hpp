@@ -83,6 +83,20 @@ struct Shape {
     static constexpr int ROW = D * W;  // scaled-increment table per step
 };
 
+// FMAs one lane issues per Chen step: for each subtree target M the path
+// partial (P-1), then the subtree levels P..M; plus the path targets.
+template <int W, int D, int P>
+constexpr int lane_fmas()
+{
+    int n = 0;
+    for (int M = P; M <= D; ++M) {
+        n += P - 1;
+        for (int K = P; K <= M; ++K) n += int(ipow(W, K - P));
+    }
+    for (int M = 1; M < P; ++M) n += M;
+    return n;
+}
+
 // Fills row[(d-1)*W + a] = x[a] * (1/d) for d = 1..D.
 template <typename T, int W, int D>
 RPY_HD inline void build_row(T* row, const T* x)
