@@ -325,6 +325,40 @@ def gram_leg(torch, steps, warmup):
     return res
 
 
+def graph_leg(torch, reps=200):
+    """A launch-bound call: signatures of 16 short streams (width 5, depth 5,
+    lengths 2..8), timed as eager calls and as replays of a HIP graph
+    captured once (the length sort, the queue reset and the persistent
+    launch)."""
+    from roughpy_amd import _capi, _device
+    w, d, n = 5, 5, 16
+    rng = np.random.default_rng(SEED + 50)
+    lens = rng.integers(2, 9, size=n)
+    offs = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).cuda()
+    inc = torch.from_numpy((rng.normal(size=(int(lens.sum()), w)) * 0.1).astype(np.float32)).cuda()
+    tsize = int(_capi.lib().rpy_tensor_size(w, d))
+    out = torch.empty((n, tsize), device="cuda")
+    lib = _capi.lib()
+
+    def run():
+        _capi.check(lib.rpy_stream_signature_batch(out.data_ptr(), inc.data_ptr(),
+                                                   offs.data_ptr(), n, w, d, _capi.RPY_F32,
+                                                   _device.stream_handle()))
+    eager_ms = time_kernel(run, reps, 5, torch)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run()
+    graph_ms = time_kernel(g.replay, reps, 5, torch)
+    return {"workload": f"signatures of {n} streams, width {w}, depth {d}, lengths 2..8",
+            "eager_us_per_call": eager_ms * 1e3, "graph_us_per_replay": graph_ms * 1e3}
+
+
 def cpu_baseline(inc, offs, target_seconds=10.0):
     """Reference CPU path on a bounded sample: the reference's own
     roughpy_compute ft_fmexp per increment (oracle/_ref/libref.so, built from
@@ -365,9 +399,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="dyadic,fma,backward,gram",
+    ap.add_argument("--legs", default="dyadic,fma,backward,gram,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, backward, gram; empty for none)")
+                         "(dyadic, fma, backward, gram, graph; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -476,6 +510,8 @@ def main():
         line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
     if "gram" in legs:
         line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
+    if "graph" in legs:
+        line["graph_replay"] = graph_leg(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(inc, offs)
     print(json.dumps(line), flush=True)

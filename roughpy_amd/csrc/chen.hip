@@ -118,8 +118,10 @@ void chen_persistent_kernel(
         const int slot = active ? int(item / NP - si_first) : 0;
         const int q = active ? int(item % NP) : 0;
         const int len = active ? int(slot_len[slot]) : 0;
-        // streams are sorted by length, longest first: slot 0 is the longest
-        const int maxlen = int(slot_len[0]);
+        // the chunk runs as long as its longest stream (with the length sort,
+        // slot 0)
+        int maxlen = 0;
+        for (int i = 0; i < n_slots; ++i) maxlen = max(maxlen, int(slot_len[i]));
 
         int u[P];
         {
@@ -399,14 +401,21 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
     rc = RPY_OK;
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "length prep launch failed: %s", hipGetErrorString(err));
-    if (!rc) {
+    // Longest-first order only matters when the chunks run in more than one
+    // round; a batch whose waves are all resident at once (a few per CU)
+    // keeps the input order and skips the sort's launches.
+    const int np = (dtype == RPY_F64 || dtype == RPY_F32_COMPENSATED) ? e->p64 : e->p32;
+    const int64_t lanes = int64_t(n_streams) * int64_t(ipow(width, np));
+    const bool sort = (lanes + 63) / 64 > 1024;
+    if (!rc && sort) {
         err = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys_in, keys_out, vals_in,
                                              vals_out, n, 0, 32, st);
         if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
     }
     if (!rc) {
         Launcher f = (dtype == RPY_F64) ? e->f64 : (dtype == RPY_F32) ? e->f32 : e->f32c;
-        rc = f(out, increments, begins, ends, vals_out, n_streams, b, counter, st);
+        rc = f(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b, counter,
+               st);
     }
     hipError_t ferr = hipFreeAsync(ws, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ferr));
