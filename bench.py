@@ -325,6 +325,43 @@ def gram_leg(torch, steps, warmup):
     return res
 
 
+def shuffle_leg(torch, steps, warmup):
+    """Shuffle product C += A (shuffle) B, width 4, depth 6 (5461 coefficients),
+    batch 16384, uniform[-1, 1] coefficients.  Algorithmic work: every output
+    word of degree n sums 2^n products, 2 * sum_n (2W)^n flop per element;
+    traffic read A, B, C and write C."""
+    from roughpy_amd import _capi, _device
+    from roughpy_amd.bases import TensorBasis
+    w, d, batch = 4, 6, 16384
+    tb = TensorBasis(w, d)
+    n = tb.size()
+    cs = tb.c_struct()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 60)
+    A32 = torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1
+    B32 = torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1
+    terms = sum((2 * w) ** k for k in range(d + 1))
+    res = {"workload": f"C += A (shuffle) B, width {w}, depth {d} ({n} coefficients), "
+                       f"batch {batch}", "products_per_element": terms}
+    lib = _capi.lib()
+    for name, code, A, B in (("f64", _capi.RPY_F64, A32.double(), B32.double()),
+                             ("f32", _capi.RPY_F32, A32, B32)):
+        C = torch.zeros_like(A)
+
+        def fn():
+            _capi.check(lib.rpy_dense_st_fma(C.data_ptr(), A.data_ptr(), B.data_ptr(), code, cs,
+                                             batch, n, n, n, _device.stream_handle()))
+        ms = time_kernel(fn, steps, warmup, torch)
+        tf = 2.0 * terms * batch / (ms * 1e-3) / 1e12
+        gbs = 4.0 * n * A.element_size() * batch / (ms * 1e-3) / 1e9
+        res[name] = {"ms_per_step": ms, "tflops": tf, "valu_frac": tf / PEAK_TFLOPS[name],
+                     "hbm_gb_s": gbs}
+        del C
+    del A32, B32
+    torch.cuda.empty_cache()
+    return res
+
+
 def graph_leg(torch, reps=200):
     """A launch-bound call: signatures of 16 short streams (width 5, depth 5,
     lengths 2..8), timed as eager calls and as replays of a HIP graph
@@ -399,9 +436,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="dyadic,fma,backward,gram,graph",
+    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, backward, gram, graph; empty for none)")
+                         "(dyadic, fma, backward, gram, shuffle, graph; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -510,6 +547,8 @@ def main():
         line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
     if "gram" in legs:
         line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
+    if "shuffle" in legs:
+        line["shuffle_w4_d6"] = shuffle_leg(torch, max(3, args.steps // 2), 1)
     if "graph" in legs:
         line["graph_replay"] = graph_leg(torch)
     if not args.no_cpu_baseline:

@@ -167,6 +167,24 @@ int rpy_dense_ft_adjoint_left_mul(void *out, const void *op, const void *arg,
                                   int64_t op_stride, int64_t arg_stride,
                                   void *stream);
 
+/* out += lhs (shuffle) rhs at full depth (1..12); replaces dense_st_fma
+ * (roughpy/compute/_src/dense_basic.cpp:449-534, kernel
+ * roughpy_compute/dense/basic/shuffle_tensor_product.hpp:24-85).  Same
+ * mask order as the reference, so float results round the same way. */
+int rpy_dense_st_fma(void *out, const void *lhs, const void *rhs,
+                     rpy_dtype dtype, const rpy_tensor_basis *basis,
+                     int64_t batch, int64_t out_stride, int64_t lhs_stride,
+                     int64_t rhs_stride, void *stream);
+
+/* out += adjoint of shuffle multiplication by op, applied to arg:
+ * <out, x> = <arg, op (shuffle) x>; the kernel of
+ * roughpy_compute/dense/basic/shuffle_tensor_adjoint_mul.hpp:13-70 (the
+ * reference exposes it to C++ only). */
+int rpy_dense_st_adj_mul(void *out, const void *op, const void *arg,
+                         rpy_dtype dtype, const rpy_tensor_basis *basis,
+                         int64_t batch, int64_t out_stride, int64_t op_stride,
+                         int64_t arg_stride, void *stream);
+
 /* out = exp(arg) (arg[0] ignored); replaces dense_ft_exp
  * (roughpy/compute/_src/dense_intermediate.cpp:62-102, kernel
  * free_tensor_exp.hpp:13-37). */

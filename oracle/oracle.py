@@ -62,6 +62,8 @@ def lib() -> ctypes.CDLL:
         getattr(L, f"orc_ft_log_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int]
         getattr(L, f"orc_antipode_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int]
         getattr(L, f"orc_adj_lmul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int] + [_c_int] * 6
+        getattr(L, f"orc_st_fma_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int]
+        getattr(L, f"orc_st_adj_mul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int]
         getattr(L, f"orc_signature_{sfx}").argtypes = [_ptr, _ptr, _c_i64, _c_int, _c_int, _ptr]
         f = getattr(L, f"orc_signature_batch_{sfx}")
         f.argtypes = [_ptr, _ptr, _ptr, _c_i64, _c_int, _c_int, _c_int]
@@ -345,6 +347,36 @@ def ft_adjoint_left_mul(op, arg, width, depth):
     n, (o, p, a) = _batched(out, op, arg)
     for e in range(n):
         f(_p(o[e]), _p(p[e]), _p(a[e]), width, 0, depth, 0, depth, 0, depth)
+    return out
+
+
+def st_fma(out, lhs, rhs, width, depth):
+    """out += lhs (shuffle) rhs in place (orc_st_fma, restating
+    shuffle_tensor_product.hpp:24-85)."""
+    L = lib()
+    f = getattr(L, f"orc_st_fma_{_sfx(out.dtype)}")
+    n, (o, l, r) = _batched(out, np.ascontiguousarray(lhs), np.ascontiguousarray(rhs))
+    for e in range(n):
+        f(_p(o[e]), _p(l[e]), _p(r[e]), width, depth)
+    return out
+
+
+def st_mul(lhs, rhs, width, depth):
+    lhs = np.ascontiguousarray(lhs)
+    out = np.zeros_like(lhs)
+    return st_fma(out, lhs, rhs, width, depth)
+
+
+def st_adj_mul(op, arg, width, depth):
+    """Adjoint of shuffle multiplication by op (orc_st_adj_mul, restating
+    shuffle_tensor_adjoint_mul.hpp:13-70)."""
+    L = lib()
+    f = getattr(L, f"orc_st_adj_mul_{_sfx(arg.dtype)}")
+    arg = np.ascontiguousarray(arg)
+    out = np.zeros_like(arg)
+    n, (o, p, a) = _batched(out, np.ascontiguousarray(op), arg)
+    for e in range(n):
+        f(_p(o[e]), _p(p[e]), _p(a[e]), width, depth)
     return out
 
 

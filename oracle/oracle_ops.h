@@ -182,6 +182,83 @@ void SFX(orc_adj_lmul)(REAL *out, const REAL *op, const REAL *arg, int w,
     }
 }
 
+/* Shuffle product out += lhs (sh) rhs at full depth D, restating the v1
+ * kernel of roughpy_compute/dense/basic/shuffle_tensor_product.hpp:24-85:
+ * out[0] += lhs[0] rhs[0]; then for each output word (degree 1..D, letters
+ * unpacked least significant first) the sum over all 2^n bit masks, bit j
+ * sending letter j (j = 0 the last letter) to the lhs word and the others to
+ * the rhs word (basis.hpp:50-61), accumulated from zero and then added. */
+void SFX(orc_st_fma)(REAL *out, const REAL *lhs, const REAL *rhs, int w, int D)
+{
+    int64_t db[ORC_MAX_DEPTH + 2];
+    int letters[ORC_MAX_DEPTH];
+    orc_degree_begin(w, D, db);
+    out[0] += lhs[0] * rhs[0];
+    for (int n = 1; n <= D; ++n) {
+        int64_t size = db[n + 1] - db[n];
+        for (int64_t i = 0; i < size; ++i) {
+            int64_t t = i;
+            for (int d = 0; d < n; ++d) {
+                letters[d] = (int)(t % w);
+                t /= w;
+            }
+            REAL acc = 0;
+            for (uint64_t mask = 0; mask < ((uint64_t)1 << n); ++mask) {
+                int64_t li = 0, ri = 0;
+                int ld = 0, rd = 0;
+                for (int d = n - 1; d >= 0; --d) {
+                    if ((mask >> d) & 1) {
+                        ++ld;
+                        li = li * w + letters[d];
+                    } else {
+                        ++rd;
+                        ri = ri * w + letters[d];
+                    }
+                }
+                acc += lhs[db[ld] + li] * rhs[db[rd] + ri];
+            }
+            out[db[n] + i] += acc;
+        }
+    }
+}
+
+/* Adjoint of shuffle multiplication by `op`, restating
+ * roughpy_compute/dense/basic/shuffle_tensor_adjoint_mul.hpp:13-70: for
+ * every word u of arg (degree 0..D) and every bit mask, the masked letters
+ * form the op word and the rest the out word:
+ *   out[rest] += arg[u] * op[masked]. */
+void SFX(orc_st_adj_mul)(REAL *out, const REAL *op, const REAL *arg, int w, int D)
+{
+    int64_t db[ORC_MAX_DEPTH + 2];
+    int letters[ORC_MAX_DEPTH];
+    orc_degree_begin(w, D, db);
+    for (int n = 0; n <= D; ++n) {
+        int64_t size = db[n + 1] - db[n];
+        for (int64_t i = 0; i < size; ++i) {
+            int64_t t = i;
+            for (int d = 0; d < n; ++d) {
+                letters[d] = (int)(t % w);
+                t /= w;
+            }
+            REAL scalar = arg[db[n] + i];
+            for (uint64_t mask = 0; mask < ((uint64_t)1 << n); ++mask) {
+                int64_t pi = 0, oi = 0;
+                int pd = 0, od = 0;
+                for (int d = n - 1; d >= 0; --d) {
+                    if ((mask >> d) & 1) {
+                        ++pd;
+                        pi = pi * w + letters[d];
+                    } else {
+                        ++od;
+                        oi = oi * w + letters[d];
+                    }
+                }
+                out[db[od] + oi] += scalar * op[db[pd] + pi];
+            }
+        }
+    }
+}
+
 /* Signature of a piecewise-linear stream by Chen's identity, restating
  * algebra/src/lite_context.h:545-563: result = 1; for each increment row,
  * result = result * exp(l2t(row)), the exponential applied with the fmexp

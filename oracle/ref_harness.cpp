@@ -12,6 +12,8 @@
 //   intermediate::ft_exp      roughpy_compute/dense/intermediate/free_tensor_exp.hpp:13-37
 //   intermediate::ft_fmexp    roughpy_compute/dense/intermediate/free_tensor_fmexp.hpp:15-40
 //   intermediate::ft_log      roughpy_compute/dense/intermediate/free_tensor_log.hpp:12-40
+//   basic::st_fma             roughpy_compute/dense/basic/shuffle_tensor_product.hpp:24-85
+//   basic::st_adj_mul         roughpy_compute/dense/basic/shuffle_tensor_adjoint_mul.hpp:13-70
 // The batched signature loop restates algebra/src/lite_context.h:545-563
 // (result = 1; result = result * exp(row) per increment) on top of ft_fmexp.
 
@@ -28,6 +30,8 @@
 #include "roughpy_compute/dense/intermediate/free_tensor_exp.hpp"
 #include "roughpy_compute/dense/intermediate/free_tensor_fmexp.hpp"
 #include "roughpy_compute/dense/intermediate/free_tensor_log.hpp"
+#include "roughpy_compute/dense/basic/shuffle_tensor_product.hpp"
+#include "roughpy_compute/dense/basic/shuffle_tensor_adjoint_mul.hpp"
 
 namespace rc = rpy::compute;
 
@@ -132,6 +136,30 @@ void adj_lmul_batch(T* out, const T* op, const T* arg, int w, int d, int64_t n)
 }
 
 template <typename T>
+void st_fma_batch(T* out, const T* lhs, const T* rhs, int w, int d, int64_t n)
+{
+    Basis b(w, d);
+    for (int64_t e = 0; e < n; ++e) {
+        View<T> o(out + e * b.size(), b.tb, 0, d);
+        View<T> l(const_cast<T*>(lhs) + e * b.size(), b.tb, 0, d);
+        View<T> r(const_cast<T*>(rhs) + e * b.size(), b.tb, 0, d);
+        rc::basic::st_fma(o, l, r);
+    }
+}
+
+template <typename T>
+void st_adj_mul_batch(T* out, const T* op, const T* arg, int w, int d, int64_t n)
+{
+    Basis b(w, d);
+    for (int64_t e = 0; e < n; ++e) {
+        View<T> o(out + e * b.size(), b.tb, 0, d);
+        View<T> p(const_cast<T*>(op) + e * b.size(), b.tb, 0, d);
+        View<T> a(const_cast<T*>(arg) + e * b.size(), b.tb, 0, d);
+        rc::basic::st_adj_mul(o, p, a);
+    }
+}
+
+template <typename T>
 void signature_range(T* out, const T* incr, const int64_t* offsets,
                      int64_t first, int64_t last, int w, int d)
 {
@@ -197,6 +225,12 @@ void signature_batch(T* out, const T* incr, const int64_t* offsets,
     extern "C" void ref_adj_lmul_##SFX(T* o, const T* p, const T* a, int w,   \
                                        int d, int64_t n)                       \
     { adj_lmul_batch(o, p, a, w, d, n); }                                      \
+    extern "C" void ref_st_fma_##SFX(T* o, const T* l, const T* r, int w,     \
+                                     int d, int64_t n)                         \
+    { st_fma_batch(o, l, r, w, d, n); }                                        \
+    extern "C" void ref_st_adj_mul_##SFX(T* o, const T* p, const T* a, int w, \
+                                         int d, int64_t n)                     \
+    { st_adj_mul_batch(o, p, a, w, d, n); }                                    \
     extern "C" void ref_signature_batch_##SFX(T* o, const T* incr,            \
                                               const int64_t* off, int64_t ns,  \
                                               int w, int d, int nt)            \

@@ -106,6 +106,10 @@ SIGNATURES = {
                                                 ctypes.c_int, _vp]),
     "rpy_debug_vjp_host_emulation": (ctypes.c_int, [_p_dbl, _p_dbl, _i64, _i32, _i32, _i32,
                                                     _p_dbl, _p_dbl]),
+    "rpy_dense_st_fma": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
+                                        _i64, _vp]),
+    "rpy_dense_st_adj_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
+                                            _i64, _vp]),
     "rpy_dense_ft_adjoint_right_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64,
                                                       _i64, _i64, _i64, _vp]),
 }

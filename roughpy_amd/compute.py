@@ -21,7 +21,8 @@ __all__ = [
     "TensorBasis", "LieBasis", "SparseMatrix", "DenseFreeTensor", "DenseShuffleTensor",
     "DenseLie", "FreeTensor", "ShuffleTensor", "Lie", "check_lie_basis_data", "ft_fma",
     "ft_mul", "ft_inplace_mul", "antipode", "ft_exp", "ft_log", "ft_fmexp",
-    "ft_adjoint_left_mul", "lie_to_tensor", "tensor_to_lie",
+    "ft_adjoint_left_mul", "lie_to_tensor", "tensor_to_lie", "st_fma", "st_mul",
+    "st_adj_mul",
 ]
 
 
@@ -272,6 +273,53 @@ def ft_adjoint_left_mul(op: FreeTensor, arg: ShuffleTensor) -> ShuffleTensor:
         a.stride, _device.stream_handle()))
     res = res.reshape(_shape(arg.data)[:-1] + (size,))
     return ShuffleTensor(_wrap_result(res, arg.data), arg.basis)
+
+
+def st_fma(a: ShuffleTensor, b: ShuffleTensor, c: ShuffleTensor) -> ShuffleTensor:
+    """Shuffle tensor fused multiply-add: a += b (shuffle) c, in place on a.data;
+    returns a (roughpy/compute/__init__.py:203-216)."""
+    _check_basis_compat(a.basis, b.basis, c.basis)
+    dt = _device.np_dtype(a.data)
+    code = _dtype_code(dt)
+    _check_leading(a.data, b.data, c.data)
+    size = a.basis.size()
+    out = _Operand(a.data, dt, size, "out")
+    lhs = _padded(b.data, size, dt, "lhs")
+    rhs = _padded(c.data, size, dt, "rhs")
+    _capi.check(_capi.lib().rpy_dense_st_fma(
+        out.ptr, lhs.ptr, rhs.ptr, code, a.basis.c_struct(), out.batch, out.stride, lhs.stride,
+        rhs.stride, _device.stream_handle()))
+    out.finish()
+    return a
+
+
+def st_mul(lhs: ShuffleTensor, rhs: ShuffleTensor) -> ShuffleTensor:
+    """Shuffle product (roughpy/compute/__init__.py:219-232)."""
+    _check_basis_compat(lhs.basis, rhs.basis)
+    dt = _device.np_dtype(lhs.data)
+    _dtype_code(dt)
+    res = _new_like(_shape(lhs.data), dt)
+    st_fma(ShuffleTensor(res, lhs.basis), lhs, rhs)
+    return ShuffleTensor(_wrap_result(res, lhs.data), lhs.basis)
+
+
+def st_adj_mul(op: ShuffleTensor, arg: FreeTensor) -> FreeTensor:
+    """Adjoint of shuffle multiplication by `op`, applied to `arg`:
+    <st_adj_mul(op, arg), x> = <arg, op (shuffle) x>
+    (roughpy_compute/dense/basic/shuffle_tensor_adjoint_mul.hpp:13-70)."""
+    _check_basis_compat(op.basis, arg.basis)
+    dt = _device.np_dtype(arg.data)
+    code = _dtype_code(dt)
+    _check_leading(op.data, arg.data)
+    size = arg.basis.size()
+    a = _Operand(arg.data, dt, size, "arg")
+    o = _padded(op.data, size, dt, "op")
+    res = _new_like((a.batch, size), dt)
+    _capi.check(_capi.lib().rpy_dense_st_adj_mul(
+        res.data_ptr(), o.ptr, a.ptr, code, arg.basis.c_struct(), a.batch, size, o.stride,
+        a.stride, _device.stream_handle()))
+    res = res.reshape(_shape(arg.data)[:-1] + (size,))
+    return FreeTensor(_wrap_result(res, arg.data), arg.basis)
 
 
 def _apply_map(mat: SparseMatrix, x, dt, out_cols_basis_size, scale):

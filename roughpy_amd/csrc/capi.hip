@@ -204,6 +204,20 @@ int rpy_dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rp
     return dense_ft_adjoint_left_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
 }
 
+int rpy_dense_st_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ls,
+                     int64_t rs, void* stream)
+{
+    return dense_st_fma(out, lhs, rhs, dtype, basis, batch, os, ls, rs, stream);
+}
+
+int rpy_dense_st_adj_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                         const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ps,
+                         int64_t as, void* stream)
+{
+    return dense_st_adj_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+}
+
 int rpy_dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
                      int64_t batch, int64_t os, int64_t as, void* stream)
 {

@@ -63,3 +63,15 @@ int gram_bf16x3(float* G, const float* S, int64_t n, int64_t k, int64_t lds, int
                 void* stream);
 
 }  // namespace rpy_amd
+
+namespace rpy_amd {
+
+// shuffle.hip
+int dense_st_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                 const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ls,
+                 int64_t rs, void* stream);
+int dense_st_adj_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ps,
+                     int64_t as, void* stream);
+
+}  // namespace rpy_amd

@@ -63,8 +63,13 @@ def main(path=os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vect
             getattr(R, f"ref_antipode_{sfx}")(p(ap), p(c), w, d, n)
             adj = np.zeros_like(a)
             getattr(R, f"ref_adj_lmul_{sfx}")(p(adj), p(a), p(c), w, d, n)
+            st = c.copy()
+            getattr(R, f"ref_st_fma_{sfx}")(p(st), p(a), p(b), w, d, n)
+            st_adj = np.zeros_like(a)
+            getattr(R, f"ref_st_adj_mul_{sfx}")(p(st_adj), p(a), p(c), w, d, n)
             for name, arr in dict(a=a, b=b, c=c, x=x, fma=fma, inplace_mul=imul, exp=ex,
-                                  fmexp=fmx, log=lg, antipode=ap, adj_lmul=adj).items():
+                                  fmexp=fmx, log=lg, antipode=ap, adj_lmul=adj, st_fma=st,
+                                  st_adj_mul=st_adj).items():
                 out[f"{key}/{name}"] = arr
             lens = np.array([0, 5, 17, 1], dtype=np.int64)
             offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)

@@ -123,6 +123,10 @@ def test_oracle_matches_reference_kernels(key):
     assert_allclose(O.ft_adjoint_left_mul(g["a"], g["c"], w, d), g["adj_lmul"], rtol=tol, atol=tol)
     sig = O.signature_batch(g["sig_increments"], g["sig_offsets"], w, d)
     assert_allclose(sig, g["signature"], rtol=tol, atol=tol)
+    st = g["c"].copy()
+    O.st_fma(st, g["a"], g["b"], w, d)
+    assert_array_equal(st, g["st_fma"])          # same loop order: exact
+    assert_array_equal(O.st_adj_mul(g["a"], g["c"], w, d), g["st_adj_mul"])
 
 
 # --- (c) mathematics ----------------------------------------------------------
@@ -347,3 +351,102 @@ def test_signature_derivative_composition(rng):
     for E, P in zip(Es, Ps):
         exp_ = O.ft_mul(exp_, E, w, d) + O.signature_derivative([(E, P)], w, d)
     assert_allclose(whole, exp_, atol=1e-13)
+
+
+# --- shuffle product ----------------------------------------------------------
+
+def _flat(w, *letters):
+    """Flat index of a word (0-based letters), first letter most significant."""
+    db = O.tensor_degree_begin(w, len(letters))
+    i = 0
+    for a in letters:
+        i = i * w + a
+    return int(db[len(letters)]) + i
+
+
+def test_shuffle_goldens():
+    """tests/compute/test_shuffle.py:58-112: letter (sh) letter, and a degree-2
+    word (sh) a letter, at width 4, depth 3."""
+    w, d = 4, 3
+    size = O.tensor_size(w, d)
+    lhs = np.zeros(size)
+    rhs = np.zeros(size)
+    lhs[_flat(w, 0)] = 1.0
+    rhs[_flat(w, 1)] = 1.0
+    expected = np.zeros(size)
+    expected[_flat(w, 0, 1)] = 1.0
+    expected[_flat(w, 1, 0)] = 1.0
+    assert_array_equal(O.st_mul(lhs, rhs, w, d), expected)
+    lhs = np.zeros(size)
+    rhs = np.zeros(size)
+    lhs[_flat(w, 0, 1)] = 1.0
+    rhs[_flat(w, 2)] = 1.0
+    expected = np.zeros(size)
+    for word in [(0, 1, 2), (0, 2, 1), (2, 0, 1)]:
+        expected[_flat(w, *word)] = 1.0
+    assert_array_equal(O.st_mul(lhs, rhs, w, d), expected)
+
+
+def test_shuffle_commutative_with_unit(rng):
+    """tests/compute/test_shuffle.py:36-55."""
+    w, d = 4, 3
+    a = rng.uniform(-1, 1, size=O.tensor_size(w, d))
+    b = rng.uniform(-1, 1, size=O.tensor_size(w, d))
+    assert_allclose(O.st_mul(a, b, w, d), O.st_mul(b, a, w, d), atol=1e-14)
+    unit = np.zeros_like(a)
+    unit[0] = 1.0
+    assert_allclose(O.st_mul(a, unit, w, d), a, atol=1e-15)
+    assert_allclose(O.st_mul(unit, a, w, d), a, atol=1e-15)
+
+
+def _shuffles(u, v):
+    """All interleavings of the words u and v, by the recursion
+    ua (sh) vb = (u (sh) vb) a + (ua (sh) v) b."""
+    if not u:
+        return [v]
+    if not v:
+        return [u]
+    return [x + u[-1:] for x in _shuffles(u[:-1], v)] + [x + v[-1:] for x in _shuffles(u, v[:-1])]
+
+
+def test_shuffle_matches_the_recursive_definition(rng):
+    """Brute force by the recursive definition of the shuffle of words,
+    expanded bilinearly, with exact small-integer coefficients."""
+    w, d = 2, 4
+    size = O.tensor_size(w, d)
+    a = rng.integers(-3, 4, size=size).astype(np.float64)
+    b = rng.integers(-3, 4, size=size).astype(np.float64)
+    ref = np.zeros(size)
+    words = [()] + [wd for k in range(1, d + 1) for wd in product(range(w), repeat=k)]
+    for u in words:
+        for v in words:
+            if len(u) + len(v) > d:
+                continue
+            for x in _shuffles(list(u), list(v)):
+                ref[_flat(w, *x)] += a[_flat(w, *u)] * b[_flat(w, *v)]
+    assert_array_equal(O.st_mul(a, b, w, d), ref)
+
+
+def test_shuffle_identity_on_signatures(rng):
+    """<S, x (sh) y> = <S, x> <S, y> for a group-like S (a signature) when
+    deg x + deg y <= depth."""
+    w, d = 3, 5
+    S = O.signature(rng.normal(size=(9, w)) / 3, w, d)
+    db = O.tensor_degree_begin(w, d)
+    x = np.zeros(O.tensor_size(w, d))
+    y = np.zeros_like(x)
+    x[:db[3]] = rng.normal(size=int(db[3]))      # degrees 0..2
+    y[:db[4]] = rng.normal(size=int(db[4]))      # degrees 0..3
+    assert abs(np.dot(S, O.st_mul(x, y, w, d)) - np.dot(S, x) * np.dot(S, y)) < 1e-12
+
+
+def test_shuffle_adjoint_is_the_adjoint(rng):
+    """<st_adj_mul(op, arg), x> = <arg, op (sh) x> over a basis of x."""
+    w, d = 3, 3
+    size = O.tensor_size(w, d)
+    op = rng.normal(size=size)
+    arg = rng.normal(size=size)
+    adj = O.st_adj_mul(op, arg, w, d)
+    E = np.eye(size)
+    brute = np.array([np.dot(arg, O.st_mul(op, E[i], w, d)) for i in range(size)])
+    assert_allclose(adj, brute, atol=1e-13)
