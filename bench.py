@@ -354,8 +354,15 @@ def shuffle_leg(torch, steps, warmup):
         ms = time_kernel(fn, steps, warmup, torch)
         tf = 2.0 * terms * batch / (ms * 1e-3) / 1e12
         gbs = 4.0 * n * A.element_size() * batch / (ms * 1e-3) / 1e9
+
+        def adj():
+            _capi.check(lib.rpy_dense_st_adj_mul(C.data_ptr(), A.data_ptr(), B.data_ptr(), code,
+                                                 cs, batch, n, n, n, _device.stream_handle()))
+        ms_adj = time_kernel(adj, max(1, steps // 3), 1, torch)
         res[name] = {"ms_per_step": ms, "tflops": tf, "valu_frac": tf / PEAK_TFLOPS[name],
-                     "hbm_gb_s": gbs}
+                     "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+                     "adjoint_ms_per_step": ms_adj,
+                     "adjoint_tflops": 2.0 * terms * batch / (ms_adj * 1e-3) / 1e12}
         del C
     del A32, B32
     torch.cuda.empty_cache()

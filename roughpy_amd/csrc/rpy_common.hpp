@@ -52,7 +52,14 @@ struct DevBasis {
 // DevBasis; returns RPY_OK or an error code with the message set.
 int make_basis(const rpy_tensor_basis* b, DevBasis& out);
 
-constexpr RPY_HD int64_t ipow(int64_t b, int e) { return e <= 0 ? 1 : b * ipow(b, e - 1); }
+// b^e by a loop (not recursion, so device code inlines it wherever it is not
+// folded at compile time)
+constexpr RPY_HD inline int64_t ipow(int64_t b, int e)
+{
+    int64_t r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
 
 }  // namespace rpy_amd
 

@@ -74,7 +74,7 @@ def test_shuffle_batched_vs_oracle(gpu, rng, w, d, dtype):
     out = rc.ShuffleTensor(c.copy(), tb)
     rc.st_fma(out, rc.ShuffleTensor(a, tb), rc.ShuffleTensor(b, tb))
     assert _level_err(out.data, ref, w, d) <= tol
-    if size <= 1500:
+    if size <= 6000:
         ref_adj = O.st_adj_mul(a.astype(np.float64), c.astype(np.float64), w, d)
         got = rc.st_adj_mul(rc.ShuffleTensor(a, tb), rc.FreeTensor(c, tb)).data
         assert _level_err(got, ref_adj, w, d) <= tol
@@ -89,3 +89,23 @@ def test_shuffle_adjoint_identity_on_gpu(gpu, rng):
     adj = rc.st_adj_mul(rc.ShuffleTensor(op, tb), rc.FreeTensor(arg, tb)).data
     sh = rc.st_mul(rc.ShuffleTensor(op, tb), rc.ShuffleTensor(x, tb)).data
     assert abs(np.dot(adj, x) - np.dot(arg, sh)) <= 1e-12 * max(1.0, abs(np.dot(arg, sh)))
+
+
+@pytest.mark.parametrize("w,d,dtype", [(10, 2, np.float64), (9, 3, np.float32),
+                                       (8, 6, np.float64), (4, 8, np.float32),
+                                       (2, 12, np.float64)])
+def test_shuffle_generic_and_global_paths(gpu, rng, w, d, dtype):
+    """Widths 9 and 10 (outside the fixed-shape kernels), fixed shapes whose
+    operands do not fit LDS (the global-memory variants: width 8 depth 6
+    float64, width 4 depth 8 float32) and the deepest fixed shape."""
+    tb = rc.TensorBasis(w, d)
+    size = tb.size()
+    a = rng.uniform(-1, 1, size=(2, size)).astype(dtype)
+    b = rng.uniform(-1, 1, size=(2, size)).astype(dtype)
+    tol = 1e-12 if dtype == np.float64 else 5e-5
+    ref = O.st_mul(a.astype(np.float64), b.astype(np.float64), w, d)
+    got = rc.st_mul(rc.ShuffleTensor(a, tb), rc.ShuffleTensor(b, tb)).data
+    assert _level_err(got, ref, w, d) <= tol
+    ref_adj = O.st_adj_mul(a.astype(np.float64), b.astype(np.float64), w, d)
+    got_adj = rc.st_adj_mul(rc.ShuffleTensor(a, tb), rc.FreeTensor(b, tb)).data
+    assert _level_err(got_adj, ref_adj, w, d) <= tol
