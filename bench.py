@@ -369,6 +369,53 @@ def shuffle_leg(torch, steps, warmup):
     return res
 
 
+def simplify_leg(torch, steps):
+    """Stream.simplify of one stream of 2^20 unit-spaced float32 increments
+    (width 5, depth 5, N(0, 1/64) entries) over a partition into 16384 equal
+    pieces of 64 increments: one native row selection, one gather on the GPU
+    and one log-signature launch (Chen steps, log, Hall projection).  Reports
+    the end-to-end rate and the log-signature launch alone, whose arithmetic
+    is the headline's Chen-step count per increment."""
+    import time
+
+    from roughpy_amd import _capi, _device
+    from roughpy_amd import streams as rs
+    from roughpy_amd.bases import LieBasis
+    n, pieces = 1 << 20, 16384
+    rng = np.random.default_rng(SEED + 70)
+    inc = (rng.normal(size=(n, W)) / 8.0).astype(np.float32)
+    st = rs.LieIncrementStream.from_increments(inc, depth=D)
+    part = rs.Partition(rs.RealInterval(0.0, float(n)),
+                        np.arange(1, pieces, dtype=np.float64) * (n // pieces))
+    st.simplify(part, 0)  # warm-up: device rows, bases, maps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.simplify(part, 0)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    mids = part.intermediates()
+    idx, offs = st._select_batch(np.array([0.0] + mids), np.array(mids + [float(n)]), 0)
+    rows = st._dev_rows.index_select(0, torch.from_numpy(idx).cuda())
+    doffs = torch.from_numpy(offs).cuda()
+    lb = LieBasis(W, D)
+    t2l = lb.get_t2l_matrix(np.float64)
+    out = torch.empty((pieces, lb.size()), dtype=torch.float64, device="cuda")
+    lib = _capi.lib()
+
+    def fn():
+        _capi.check(lib.rpy_stream_log_signature_batch(
+            out.data_ptr(), rows.data_ptr(), doffs.data_ptr(), pieces, W, D, _capi.RPY_F32,
+            t2l.device_struct(), _device.stream_handle()))
+    ms_k = time_kernel(fn, steps, 1, torch)
+    tf = 2.0 * FMA_PER_STEP * n / (ms_k * 1e-3) / 1e12
+    return {"workload": f"simplify a stream of {n} increments (width {W}, depth {D}, float32) "
+                        f"into {pieces} pieces of {n // pieces}",
+            "ms_per_call": ms, "pieces_per_s": pieces / (ms * 1e-3),
+            "log_signature_launch_ms": ms_k, "launch_share": ms_k / ms,
+            "launch_tflops": tf, "launch_valu_frac": tf / PEAK_TFLOPS["f32"]}
+
+
 def graph_leg(torch, reps=200):
     """A launch-bound call: signatures of 16 short streams (width 5, depth 5,
     lengths 2..8), timed as eager calls and as replays of a HIP graph
@@ -443,9 +490,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,graph",
+    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,simplify,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, backward, gram, shuffle, graph; empty for none)")
+                         "(dyadic, fma, backward, gram, shuffle, simplify, graph; empty for "
+                         "none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -556,6 +604,8 @@ def main():
         line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
     if "shuffle" in legs:
         line["shuffle_w4_d6"] = shuffle_leg(torch, max(3, args.steps // 2), 1)
+    if "simplify" in legs:
+        line["simplify_w5_d5"] = simplify_leg(torch, max(3, args.steps // 2))
     if "graph" in legs:
         line["graph_replay"] = graph_leg(torch)
     if not args.no_cpu_baseline:

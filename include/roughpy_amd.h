@@ -349,6 +349,23 @@ int rpy_stream_select_rows(const double *params, int64_t n_params,
                            int32_t query_resolution, int64_t *ranges,
                            int64_t max_ranges, int64_t *n_ranges);
 
+/* The rows of many queries at once, for Stream::simplify
+ * (streams/src/stream.cpp:232-258), which queries the stream's
+ * log-signature on every interval of a partition: query q is
+ * [infs[q], sups[q]) trimmed to the support [support_inf, support_sup)
+ * (Stream::check_support_and_trim, stream.cpp:37-52; no rows when it misses
+ * the support or is empty), dissected at query_resolution and selected as in
+ * rpy_stream_select_rows.  offsets (host, n_queries + 1) receives the CSR
+ * offsets of the selected row indices; rows (host, max_rows entries) receives
+ * the indices, or may be NULL to size the output first.  The params are
+ * checked once per call. */
+int rpy_stream_select_rows_batch(const double *params, int64_t n_params,
+                                 int32_t stream_resolution, double support_inf,
+                                 double support_sup, const double *infs,
+                                 const double *sups, int64_t n_queries,
+                                 int32_t query_resolution, int64_t *offsets,
+                                 int64_t *rows, int64_t max_rows);
+
 /* ---- test hooks (host only; no counterpart in the reference) ----------- */
 
 /* Runs the lane decomposition of the specialised signature kernel on the

@@ -16,7 +16,8 @@ Contents, each restating the reference algorithm it cites:
   dimensions, lie_to_tensor (commutator expansion) and tensor_to_lie
   (Dynkin right bracketing, solved independently by least squares);
 * the dyadic interval dissection and the stream query semantics of
-  LieIncrementStream / DyadicCachingLayer.
+  LieIncrementStream / DyadicCachingLayer;
+* Context::cbh, PiecewiseAbelianStream queries and Stream::simplify.
 
 The parity pins of every function (reference goldens, the reference
 kernels' own vectors, closed forms, invariants, exact rational brute force)
@@ -642,6 +643,93 @@ class IncrementStream:
     def log_signature(self, inf: float, sup: float, resolution: Optional[int] = None) -> np.ndarray:
         s = self.signature(inf, sup, resolution)
         return tensor_to_lie(ft_log(s, self.width, self.depth), self.width, self.depth)
+
+
+# ---------------------------------------------------------------------------
+# Piecewise abelian streams and Stream.simplify
+# ---------------------------------------------------------------------------
+
+def cbh(lies, width: int, depth: int) -> np.ndarray:
+    """Context::cbh (algebra/src/context.cpp:169-196): a single element is
+    returned as it is; otherwise the unit tensor is multiplied by exp of
+    every element in order (cbh_fallback: fmexp of lie_to_tensor) and the
+    log is mapped back to the Lie basis."""
+    lies = [np.asarray(x, dtype=np.float64) for x in lies]
+    if len(lies) == 1:
+        return lies[0].copy()
+    coll = np.zeros(tensor_size(width, depth))
+    coll[0] = 1.0
+    for x in lies:
+        coll = ft_fmexp(coll, lie_to_tensor(x, width, depth), width, depth)
+    return tensor_to_lie(ft_log(coll, width, depth), width, depth)
+
+
+class PiecewiseAbelianStream:
+    """Oracle for PiecewiseAbelianStream queried through Stream.
+
+    log_signature_impl restates streams/src/piecewise_abelian_stream.cpp:
+    155-189 branch for branch: its third test is `pb <= pb`, always true, so
+    every piece that overlaps the dyadic interval [a, b) without lying inside
+    it is scaled by to_multiplier_upper(piece, a) = (pb - a) / (pb - pa)
+    (piecewise_abelian_stream.h:53-59) and the lower-multiplier branch is
+    never reached.  Queries restate Stream::refine_interval and
+    Stream::log_signature_impl (streams/src/stream.cpp:54-123): trim to the
+    support, dissect at the resolution, drop zero results, CBH.
+    """
+
+    def __init__(self, pieces, width: int, depth: int, resolution: int, support):
+        self.pieces = [((float(a), float(b)), np.asarray(l, dtype=np.float64))
+                       for (a, b), l in pieces]
+        self.width, self.depth = width, depth
+        self.resolution = int(resolution)
+        self.support = (float(support[0]), float(support[1]))
+
+    def _impl(self, a: float, b: float) -> np.ndarray:
+        lies = []
+        for (pa, pb), lie in self.pieces:
+            if pa >= b:
+                break
+            if pb <= a:
+                pass
+            elif pa >= a and pb <= b:
+                lies.append(lie)
+            elif pb <= pb:  # the reference's condition, always true
+                lies.append(lie * ((pb - a) / (pb - pa)))
+            elif pa >= a and pb > b:  # unreachable, kept as in the reference
+                lies.append(lie * ((b - pa) / (pb - pa)))
+        return cbh(lies, self.width, self.depth)
+
+    def log_signature(self, inf: float, sup: float, resolution: Optional[int] = None):
+        lo, hi = self.support
+        dim = sum(witt_dim(self.width, k) for k in range(1, self.depth + 1))
+        if sup < lo or inf > hi:
+            return np.zeros(dim)
+        a, b = max(inf, lo), min(sup, hi)
+        if b - a == 0.0:
+            return np.zeros(dim)
+        res = resolution if resolution is not None else max(
+            self.resolution, param_to_resolution(b - a) + 2)
+        results = []
+        for k, n in to_dyadic_intervals(a, b, res):
+            di = _DI(k, n)
+            x = self._impl(di.inf(), di.sup())
+            if np.any(x != 0):
+                results.append(x)
+        return cbh(results, self.width, self.depth)
+
+    def signature(self, inf: float, sup: float, resolution: Optional[int] = None):
+        ls = self.log_signature(inf, sup, resolution)
+        return ft_exp(lie_to_tensor(ls, self.width, self.depth), self.width, self.depth)
+
+
+def simplify(stream, intervals, resolution: int) -> PiecewiseAbelianStream:
+    """Stream::simplify (streams/src/stream.cpp:232-258): the stream's
+    log-signature over every interval of the partition at `resolution`
+    becomes a piece; the new stream keeps the support and takes the
+    resolution as its default."""
+    pieces = [((a, b), stream.log_signature(a, b, resolution)) for a, b in intervals]
+    return PiecewiseAbelianStream(pieces, stream.width, stream.depth, resolution,
+                                  stream.support)
 
 
 # ---------------------------------------------------------------------------

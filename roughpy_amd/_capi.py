@@ -92,6 +92,8 @@ SIGNATURES = {
                                                ctypes.POINTER(_i32)]),
     "rpy_stream_select_rows": (ctypes.c_int, [_p_dbl, _i64, _i32, _dbl, _dbl, _i32, _p_i64,
                                               _i64, _p_i64]),
+    "rpy_stream_select_rows_batch": (ctypes.c_int, [_p_dbl, _i64, _i32, _dbl, _dbl, _p_dbl,
+                                                    _p_dbl, _i64, _i32, _p_i64, _p_i64, _i64]),
     "rpy_debug_signature_host_emulation": (ctypes.c_int, [_p_dbl, _p_dbl, _i64, _i32, _i32,
                                                           _i32]),
     "rpy_debug_signature_prefix": (_i32, [_i32, _i32, _i32]),

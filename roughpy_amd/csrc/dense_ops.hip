@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <atomic>
 
 #include "chen_core.hpp"
 #include "dense_ops.hpp"
@@ -368,6 +369,47 @@ __global__ __launch_bounds__(kBlock) void csr_map_kernel(T* __restrict__ out,
         T acc = T(0);
         for (int64_t p = offs[r]; p < offs[r + 1]; ++p) acc += (val[p] * a[idx[p]]) * scale;
         out[e * os + r] += acc;
+    }
+}
+
+// The same map with E tensors of the batch staged in LDS per workgroup: one
+// thread per row walks the row's entries once and applies each to all E
+// tensors, so the matrix is read from L2 once per E tensors and the gathers
+// hit LDS.  Same per-row order and arithmetic as csr_map_kernel.
+template <typename T, int E, int Block>
+__global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
+                                                            const T* __restrict__ arg,
+                                                            const int64_t* __restrict__ offs,
+                                                            const int64_t* __restrict__ idx,
+                                                            const T* __restrict__ val,
+                                                            int64_t rows, int64_t cols, T scale,
+                                                            int64_t batch, int64_t os, int64_t as)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* s = reinterpret_cast<T*>(smem);
+    for (int64_t e0 = int64_t(blockIdx.x) * E; e0 < batch; e0 += int64_t(gridDim.x) * E) {
+        const int ne = int(batch - e0 < E ? batch - e0 : E);
+        __syncthreads();  // the previous tensors' readers are done
+        for (int q = 0; q < ne; ++q) {
+            const T* a = arg + (e0 + q) * as;
+            for (int64_t c = threadIdx.x; c < cols; c += Block) s[q * cols + c] = a[c];
+        }
+        __syncthreads();
+        for (int64_t r = threadIdx.x; r < rows; r += Block) {
+            T acc[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) acc[q] = T(0);
+            const int64_t pe = offs[r + 1];
+            for (int64_t p = offs[r]; p < pe; ++p) {
+                const int64_t c = idx[p];
+                const T v = val[p];
+#pragma unroll
+                for (int q = 0; q < E; ++q) acc[q] += (v * s[q * cols + c]) * scale;
+            }
+#pragma unroll
+            for (int q = 0; q < E; ++q)
+                if (q < ne) out[(e0 + q) * os + r] += acc[q];
+        }
     }
 }
 
@@ -777,6 +819,57 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
     if (batch == 0 || m->rows == 0) return RPY_OK;
     if (!out || !arg || !m->offsets || (m->nnz > 0 && (!m->indices || !m->data)))
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    // tensors staged per workgroup: 8 when they fit 48 KiB of LDS (several
+    // workgroups per CU), else up to 4 in at most 128 KiB with 512 threads
+    // (the matrix is re-read once per group, so reuse beats occupancy)
+    const int64_t es = dtype == RPY_F64 ? 8 : 4;
+    const int64_t per = m->cols * es;
+    int E = 0;
+    if (per > 0 && 8 * per <= 48 * 1024)
+        E = 8;
+    else if (per > 0 && 4 * per <= 128 * 1024)
+        E = 4;
+    else if (per > 0 && 2 * per <= 128 * 1024)
+        E = 2;
+    else if (per > 0 && per <= 128 * 1024)
+        E = 1;
+    if (per == 0) E = 8;
+    if (E > 0 && batch > 1) {
+        const size_t lds = size_t(E) * size_t(per);
+        const int64_t groups = (batch + E - 1) / E;
+        dim3 grid(uint32_t(std::min<int64_t>(groups, 65536)));
+        auto go = [&](auto tag, auto ec) {
+            using T = decltype(tag);
+            constexpr int EE = decltype(ec)::value;
+            constexpr int BL = EE == 8 ? 256 : 512;
+            auto* kern = csr_map_lds_kernel<T, EE, BL>;
+            if (lds > 64 * 1024) {
+                static std::atomic<bool> reserved{false};
+                if (!reserved.load(std::memory_order_acquire)) {
+                    hipFuncSetAttribute((const void*)kern,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+                    reserved.store(true, std::memory_order_release);
+                }
+            }
+            hipLaunchKernelGGL(kern, grid, dim3(BL), lds, as_stream(stream), (T*)out,
+                               (const T*)arg, m->offsets, m->indices, (const T*)m->data, m->rows,
+                               m->cols, T(scale), batch, os, as);
+        };
+        auto by_e = [&](auto tag) {
+            switch (E) {
+                case 8: go(tag, std::integral_constant<int, 8>{}); break;
+                case 4: go(tag, std::integral_constant<int, 4>{}); break;
+                case 2: go(tag, std::integral_constant<int, 2>{}); break;
+                default: go(tag, std::integral_constant<int, 1>{}); break;
+            }
+        };
+        if (dtype == RPY_F64)
+            by_e(double{});
+        else
+            by_e(float{});
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
     dim3 grid(blocks_elementwise(batch * m->rows)), block(kBlock);
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(csr_map_kernel<double>, grid, block, 0, as_stream(stream),

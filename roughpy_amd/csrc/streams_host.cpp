@@ -197,4 +197,45 @@ int rpy_stream_select_rows(const double* params, int64_t n_params, int32_t strea
     return RPY_OK;
 }
 
+int rpy_stream_select_rows_batch(const double* params, int64_t n_params,
+                                 int32_t stream_resolution, double support_inf,
+                                 double support_sup, const double* infs, const double* sups,
+                                 int64_t n_queries, int32_t query_resolution, int64_t* offsets,
+                                 int64_t* rows, int64_t max_rows)
+{
+    if (n_queries < 0) return fail(RPY_ERR_SHAPE, "negative query count");
+    if (!offsets || (n_params > 0 && !params) || (n_queries > 0 && (!infs || !sups)))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null argument");
+    for (int64_t i = 1; i < n_params; ++i)
+        if (!(params[i - 1] < params[i]))
+            return fail(RPY_ERR_INVALID_ARGUMENT, "params must be strictly increasing");
+    std::vector<std::pair<int64_t, int64_t>> r;
+    int64_t total = 0;
+    offsets[0] = 0;
+    for (int64_t q = 0; q < n_queries; ++q) {
+        const double qi = infs[q], qs = sups[q];
+        if (!(qs < support_inf || qi > support_sup)) {
+            const double a = std::max(qi, support_inf), b = std::min(qs, support_sup);
+            if (b - a != 0.0) {
+                int rc = select_rows(params, n_params, stream_resolution, a, b,
+                                     query_resolution, r);
+                if (rc) return rc;
+                for (const auto& pr : r) {
+                    for (int64_t i = pr.first; i < pr.second; ++i) {
+                        if (rows) {
+                            if (total >= max_rows)
+                                return fail(RPY_ERR_SHAPE, "more than %lld selected rows",
+                                            (long long)max_rows);
+                            rows[total] = i;
+                        }
+                        ++total;
+                    }
+                }
+            }
+        }
+        offsets[q + 1] = total;
+    }
+    return RPY_OK;
+}
+
 }  // extern "C"

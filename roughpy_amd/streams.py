@@ -25,8 +25,8 @@ from .bases import LieBasis, TensorBasis
 from .compute import DenseFreeTensor, DenseLie
 
 __all__ = [
-    "RealInterval", "LieIncrementStream", "DyadicSignatureCache", "signature_batch",
-    "log_signature_batch",
+    "RealInterval", "Partition", "LieIncrementStream", "PiecewiseAbelianStream",
+    "DyadicSignatureCache", "signature_batch", "log_signature_batch",
     "param_to_resolution", "to_dyadic_intervals",
 ]
 
@@ -63,6 +63,177 @@ class RealInterval:
 
     def __repr__(self):
         return f"RealInterval({self.inf}, {self.sup})"
+
+
+def _bounds_of(interval):
+    """(inf, sup) of a RealInterval, a (inf, sup) pair or an object with
+    inf/sup attributes or methods."""
+    if isinstance(interval, RealInterval):
+        return interval.inf, interval.sup
+    if isinstance(interval, (tuple, list)):
+        return float(interval[0]), float(interval[1])
+    inf_ = interval.inf() if callable(interval.inf) else interval.inf
+    sup_ = interval.sup() if callable(interval.sup) else interval.sup
+    return float(inf_), float(sup_)
+
+
+class Partition(RealInterval):
+    """A clopen interval [inf, sup) cut at sorted intermediate points, mirror
+    of intervals::Partition (intervals/src/partition.cpp:44-250) and its
+    Python binding (roughpy/src/intervals/partition.cpp:51-130).
+
+    Intermediates outside [inf, sup) or equal to an end point are dropped;
+    ``p[i]`` is the i-th sub-interval, left to right."""
+
+    __slots__ = ("_mids",)
+
+    def __init__(self, interval, intermediates=()):
+        inf_, sup_ = _bounds_of(interval)
+        super().__init__(inf_, sup_)
+        mids = [float(m) for m in intermediates]
+        self._mids = sorted(m for m in mids if self._contains(m) and m != inf_ and m != sup_)
+
+    def _contains(self, p: float) -> bool:
+        return self.inf <= p < self.sup
+
+    @classmethod
+    def _raw(cls, inf_, sup_, mids):
+        obj = cls.__new__(cls)
+        RealInterval.__init__(obj, inf_, sup_)
+        obj._mids = list(mids)
+        return obj
+
+    def __len__(self):
+        return 1 + len(self._mids)
+
+    def __getitem__(self, i: int) -> RealInterval:
+        n = len(self._mids)
+        if i < 0 or i > n:
+            raise IndexError("partition index out of range")
+        if n == 0:
+            return RealInterval(self.inf, self.sup)
+        lo = self.inf if i == 0 else self._mids[i - 1]
+        hi = self.sup if i == n else self._mids[i]
+        return RealInterval(lo, hi)
+
+    def intermediates(self):
+        return list(self._mids)
+
+    def refine_midpoints(self) -> "Partition":
+        """Inserts the midpoint of every sub-interval."""
+        out, prev = [], self.inf
+        for m in self._mids:
+            out.append((m + prev) / 2.0)
+            out.append(m)
+            prev = m
+        out.append((self.sup + prev) / 2.0)
+        return Partition._raw(self.inf, self.sup, out)
+
+    def mesh(self) -> float:
+        """The smallest sub-interval length (what the reference computes,
+        intervals/src/partition.cpp:75-91)."""
+        prev, diff = self.inf, math.inf
+        for m in self._mids:
+            diff = min(diff, abs(m - prev))
+            prev = m
+        return min(diff, abs(self.sup - prev))
+
+    def insert_intermediate(self, point: float) -> None:
+        point = float(point)
+        if not self._contains(point):
+            raise ValueError("provided intermediate does not lie within the interval")
+        import bisect
+        bisect.insort_left(self._mids, point)
+
+    def merge(self, other: "Partition") -> "Partition":
+        """Union of two partitions (intervals/src/partition.cpp:135-250)."""
+        linf, lsup, rinf, rsup = self.inf, self.sup, other.inf, other.sup
+        out = []
+        if lsup < rinf:
+            if lsup != linf:
+                out.append(lsup)
+            if rinf != rsup:
+                out.append(rinf)
+        elif rsup < linf:
+            if rinf != rsup:
+                out.append(rsup)
+            if lsup != linf:
+                out.append(linf)
+        else:
+            L, R = self._mids, other._mids
+            i = j = 0
+            while i < len(L) and L[i] < rinf:
+                out.append(L[i])
+                i += 1
+            while j < len(R) and R[j] < linf:
+                out.append(R[j])
+                j += 1
+            max_inf, min_sup = max(linf, rinf), min(lsup, rsup)
+            if linf != rinf:
+                out.append(max_inf)
+                if j < len(R) and R[j] == max_inf:
+                    j += 1
+                if i < len(L) and L[i] == max_inf:
+                    i += 1
+            while i < len(L) and j < len(R):
+                if L[i] == R[j]:
+                    out.append(L[i])
+                    i += 1
+                    j += 1
+                elif L[i] < R[j]:
+                    out.append(L[i])
+                    i += 1
+                else:
+                    out.append(R[j])
+                    j += 1
+            while i < len(L) and L[i] < rsup:
+                out.append(L[i])
+                i += 1
+            while j < len(R) and R[j] < lsup:
+                out.append(R[j])
+                j += 1
+            if lsup != rsup and min_sup != max_inf:
+                out.append(min_sup)
+                if i < len(L) and L[i] == min_sup:
+                    i += 1
+                if j < len(R) and R[j] == min_sup:
+                    j += 1
+            out.extend(L[i:])
+            out.extend(R[j:])
+        return Partition._raw(min(linf, rinf), max(lsup, rsup), out)
+
+    def __eq__(self, other):
+        return (isinstance(other, Partition) and self.inf == other.inf
+                and self.sup == other.sup and self._mids == other._mids)
+
+    def __hash__(self):
+        return hash((self.inf, self.sup, tuple(self._mids)))
+
+    def __reduce__(self):
+        return (Partition._raw, (self.inf, self.sup, list(self._mids)))
+
+    def __repr__(self):
+        return f"Partition({self.inf}, {self.sup}, {self._mids})"
+
+    def __str__(self):
+        return "".join(f"[{self[i].inf}, {self[i].sup})" for i in range(len(self)))
+
+
+def _cbh_rows(lies, tb: TensorBasis, lb: LieBasis):
+    """Context::cbh over the rows of each group (algebra/src/context.cpp:
+    187-196): log(prod_j exp(l_j)) in the tensor algebra, mapped back to the
+    Lie basis.  lies: (T, M, lie_dim) GPU tensor, T factors for each of M
+    groups (zero rows are unit factors).  Returns (M, lie_dim)."""
+    from . import compute as rc
+    t = _device.torch()
+    T, M = int(lies.shape[0]), int(lies.shape[1])
+    coll = t.zeros((M, tb.size()), dtype=lies.dtype, device=lies.device)
+    coll[:, 0] = 1
+    for j in range(T):
+        X = rc.lie_to_tensor(rc.Lie(lies[j].contiguous(), lb), tb).data
+        coll = rc.ft_fmexp(rc.FreeTensor(coll, tb), rc.FreeTensor(X, tb)).data
+    logc = rc.ft_log(rc.FreeTensor(coll, tb)).data
+    return rc.tensor_to_lie(rc.FreeTensor(logc, tb), lb).data
 
 
 def _resolution_arg(res) -> Optional[int]:
@@ -437,6 +608,47 @@ class LieIncrementStream:
             result = mul(result, E) + mul(E, series)
         return DenseFreeTensor(_device.to_host(result)[0], tb)
 
+    def simplify(self, partition: Partition, resolution=None, *, depth=None,
+                 ctx=None) -> "PiecewiseAbelianStream":
+        """Stream.simplify (streams/src/stream.cpp:232-258, Python binding
+        roughpy/src/streams/stream.cpp:406-469): the log-signature of every
+        sub-interval of the partition at `resolution` (default: the stream's)
+        becomes a piece of a PiecewiseAbelianStream.  All pieces are one
+        batched log-signature launch over the covered rows."""
+        res = self.resolution if resolution is None else _resolution_arg(resolution)
+        depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
+        n = len(partition)
+        mids = partition.intermediates()
+        infs = np.array([partition.inf] + mids, dtype=np.float64)
+        sups = np.array(mids + [partition.sup], dtype=np.float64)
+        idx, offs = self._select_batch(infs, sups, res)
+        t = _device.require_gpu()
+        if self._dev_rows is None:
+            self._dev_rows = _device.to_device(np.ascontiguousarray(self._rows), self.dtype)
+        rows = self._dev_rows.index_select(0, t.from_numpy(idx).to(self._dev_rows.device))
+        lb = LieBasis(self.width, depth)
+        lies = log_signature_batch(rows, offs, self.width, depth, dtype=self.dtype, lie_basis=lb)
+        del n
+        return PiecewiseAbelianStream(infs, sups, lies, self.width, depth, res, self.support)
+
+    def _select_batch(self, infs, sups, resolution):
+        """Row indices (CSR) that each query [infs[q], sups[q]) covers at
+        `resolution`, one native call."""
+        L = _capi.lib()
+        params = np.ascontiguousarray(self._params, dtype=np.float64)
+        nq = int(infs.shape[0])
+        offs = np.zeros(nq + 1, dtype=np.int64)
+        lo, hi = self.support
+        _capi.check(L.rpy_stream_select_rows_batch(
+            _capi.dblp(params), params.shape[0], self.resolution, lo, hi, _capi.dblp(infs),
+            _capi.dblp(sups), nq, int(resolution), _capi.i64p(offs), None, 0))
+        idx = np.zeros(max(1, int(offs[-1])), dtype=np.int64)
+        _capi.check(L.rpy_stream_select_rows_batch(
+            _capi.dblp(params), params.shape[0], self.resolution, lo, hi, _capi.dblp(infs),
+            _capi.dblp(sups), nq, int(resolution), _capi.i64p(offs), _capi.i64p(idx),
+            idx.shape[0]))
+        return idx[:int(offs[-1])], offs
+
     def log_signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
                       dtype=None, ctx=None) -> DenseLie:
         interval, res = self._parse(interval_or_inf, sup, resolution)
@@ -449,3 +661,133 @@ class LieIncrementStream:
         ls = log_signature_batch(np.ascontiguousarray(rows), offs, self.width, depth,
                                  dtype=self.dtype if dtype is None else dtype, lie_basis=lb)
         return DenseLie(ls[0], lb)
+
+
+class PiecewiseAbelianStream:
+    """A stream given by Lie pieces on consecutive intervals, mirror of
+    PiecewiseAbelianStream (streams/src/piecewise_abelian_stream.cpp:
+    43-189) queried through Stream (streams/src/stream.cpp:54-171).
+
+    A query [a, b) is trimmed to the support and dissected into dyadic
+    intervals at the resolution; on each dyadic interval [c, d) the pieces
+    it meets are taken in order -- a piece inside [c, d) whole, any other
+    overlapping piece [p, q) scaled by (q - c) / (q - p) (the reference's
+    branch order, piecewise_abelian_stream.cpp:170-181, sends partial
+    overlaps on either side there) -- and combined by CBH; zero results are
+    dropped and the dyadic results combined by CBH again.  Every exponential,
+    product, logarithm and basis map is a batched GPU launch.
+    """
+
+    def __init__(self, infs, sups, lies, width: int, depth: int, resolution: int,
+                 support):
+        self._inf = np.asarray(infs, dtype=np.float64).reshape(-1)
+        self._sup = np.asarray(sups, dtype=np.float64).reshape(-1)
+        t = _device.require_gpu()
+        self._lies = _device.to_device(lies, np.float64).reshape(len(self._inf), -1)
+        self.width = int(width)
+        self.depth = int(depth)
+        self.resolution = int(resolution)
+        self.support = (float(support[0]), float(support[1]))
+        self._tb = TensorBasis(self.width, self.depth)
+        self._lb = LieBasis(self.width, self.depth)
+        if int(self._lies.shape[1]) != self._lb.size():
+            raise ValueError("piece data does not match the Lie basis size")
+        del t
+
+    @property
+    def pieces(self):
+        """[(RealInterval, Lie coefficients as NumPy)] in order."""
+        host = _device.to_host(self._lies)
+        return [(RealInterval(a, b), host[i]) for i, (a, b) in enumerate(zip(self._inf, self._sup))]
+
+    def _terms(self, c: float, d: float):
+        """(piece, multiplier) pairs for the dyadic interval [c, d)."""
+        out = []
+        for i in range(len(self._inf)):
+            pa, pb = self._inf[i], self._sup[i]
+            if pa >= d:
+                break
+            if pb <= c:
+                continue
+            if pa >= c and pb <= d:
+                out.append((i, 1.0))
+            else:
+                out.append((i, (pb - c) / (pb - pa)))
+        return out
+
+    def _query(self, interval, resolution):
+        lo, hi = self.support
+        inf_, sup_ = interval
+        if sup_ < lo or inf_ > hi:
+            return None
+        a, b = max(inf_, lo), min(sup_, hi)
+        if b - a == 0.0:
+            return None
+        res = resolution if resolution is not None else max(self.resolution,
+                                                            param_to_resolution(b - a) + 2)
+        return a, b, res
+
+    def log_signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
+                      ctx=None) -> DenseLie:
+        interval, res = LieIncrementStream._parse(self, interval_or_inf, sup, resolution)
+        if depth is not None and int(depth) != self.depth:
+            raise ValueError("a piecewise abelian stream answers at the depth of its pieces")
+        t = _device.torch()
+        zero = DenseLie(np.zeros(self._lb.size()), self._lb)
+        q = self._query(interval, res)
+        if q is None:
+            return zero
+        a, b, res = q
+        groups = []
+        for k, n in to_dyadic_intervals(a, b, res):
+            c, d = math.ldexp(k, -n), math.ldexp(k + 1, -n)
+            groups.append(self._terms(c, d))
+        # dyadic results: one term is the (scaled) piece, several are CBH'd
+        results = [None] * len(groups)
+        multi = [g for g in groups if len(g) > 1]
+        if multi:
+            T = max(len(g) for g in multi)
+            X = t.zeros((T, len(multi), self._lb.size()), dtype=t.float64, device="cuda")
+            for m, g in enumerate(multi):
+                for j, (i, w) in enumerate(g):
+                    X[j, m] = self._lies[i] * w
+            cbh = _cbh_rows(X, self._tb, self._lb)
+        mi = 0
+        for gi, g in enumerate(groups):
+            if len(g) == 1:
+                i, w = g[0]
+                results[gi] = self._lies[i] * w
+            elif len(g) > 1:
+                results[gi] = cbh[mi]
+                mi += 1
+        kept = [r for r in results if r is not None and bool(t.any(r != 0))]
+        if not kept:
+            return zero
+        if len(kept) == 1:
+            return DenseLie(_device.to_host(kept[0]), self._lb)
+        X = t.stack(kept).unsqueeze(1)
+        return DenseLie(_device.to_host(_cbh_rows(X, self._tb, self._lb)[0]), self._lb)
+
+    def signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
+                  ctx=None) -> DenseFreeTensor:
+        """exp of the log-signature (Stream::signature, stream.cpp:156-171)."""
+        from . import compute as rc
+        ls = self.log_signature(interval_or_inf, sup, resolution, depth=depth, ctx=ctx)
+        t = _device.torch()
+        L = t.from_numpy(np.asarray(ls.data, dtype=np.float64).reshape(1, -1)).cuda()
+        X = rc.lie_to_tensor(rc.Lie(L, self._lb), self._tb).data
+        E = rc.ft_exp(rc.FreeTensor(X, self._tb)).data
+        return DenseFreeTensor(_device.to_host(E)[0], self._tb)
+
+    def simplify(self, partition: Partition, resolution=None, *, depth=None,
+                 ctx=None) -> "PiecewiseAbelianStream":
+        """Stream.simplify of a piecewise abelian stream: its log-signature
+        on every sub-interval becomes a piece."""
+        res = self.resolution if resolution is None else _resolution_arg(resolution)
+        n = len(partition)
+        bounds = [(partition[i].inf, partition[i].sup) for i in range(n)]
+        lies = np.stack([np.asarray(self.log_signature(RealInterval(*bd), res).data)
+                         for bd in bounds]) if n else np.zeros((0, self._lb.size()))
+        return PiecewiseAbelianStream(np.array([bd[0] for bd in bounds]),
+                                      np.array([bd[1] for bd in bounds]), lies, self.width,
+                                      self.depth, res, self.support)

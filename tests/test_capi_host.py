@@ -277,3 +277,25 @@ def test_query_kernel_dyadic_cover_matches_reference_dissection(rng):
             got = [(int(ks[i]), int(ns[i])) for i in range(n)]
             want = O.to_dyadic_intervals(a / 2.0 ** r, b / 2.0 ** r, r) if a < b else []
             assert got == want, (a, b, r)
+
+
+def test_select_rows_batch_matches_single_queries():
+    """rpy_stream_select_rows_batch (the batched selection behind simplify)
+    returns, per query, exactly the rows of the single-query selection after
+    the support trim: random uneven streams, random and edge queries."""
+    from roughpy_amd import streams as rs
+    rng = np.random.default_rng(7)
+    for res, n in ((0, 40), (4, 200), (7, 500)):
+        params = np.unique(np.round(rng.uniform(0, 5, size=n), 3))
+        inc = rng.normal(size=(params.shape[0], 2))
+        st = rs.LieIncrementStream.from_increments(inc, indices=params, depth=2, resolution=res)
+        lo, hi = st.support
+        a = rng.uniform(lo - 1, hi + 1, size=60)
+        b = a + rng.uniform(0, 2, size=60)
+        a = np.concatenate([a, [lo, hi, hi + 1, lo - 2, 2.0]])
+        b = np.concatenate([b, [hi, hi + 1, hi + 2, lo - 1, 2.0]])
+        for qres in (res, res + 3):
+            idx, offs = st._select_batch(a, b, qres)
+            for q in range(a.shape[0]):
+                single = st._select((a[q], b[q]), qres)
+                assert np.array_equal(idx[offs[q]:offs[q + 1]], single), (res, qres, q)

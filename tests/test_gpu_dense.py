@@ -157,6 +157,24 @@ def test_lie_maps_on_gpu(gpu, rng):
     assert_allclose(rc.tensor_to_lie(t2, lb).data, 0.5 * x, atol=1e-15)
 
 
+@pytest.mark.parametrize("w,d,dtype", [(5, 5, np.float64), (5, 5, np.float32),
+                                       (2, 9, np.float64), (8, 4, np.float32)])
+def test_lie_maps_batched(gpu, rng, w, d, dtype):
+    """The batched maps (tensors staged in LDS, several per workgroup, with a
+    ragged last group: 37 tensors) against the dense oracle matrices."""
+    lb = rc.LieBasis(w, d)
+    tb = rc.TensorBasis(w, d)
+    x = rng.uniform(-1, 1, size=(37, lb.size())).astype(dtype)
+    t = rc.lie_to_tensor(rc.Lie(x, lb), tb)
+    ref_t = x.astype(np.float64) @ O.l2t_dense(w, d).T
+    tol = 1e-13 if dtype == np.float64 else 2e-6
+    assert np.abs(np.asarray(t.data, np.float64) - ref_t).max() <= tol * max(1, np.abs(ref_t).max())
+    g = rng.uniform(-1, 1, size=(37, tb.size())).astype(dtype)
+    y = rc.tensor_to_lie(rc.FreeTensor(g, tb), lb, scale_factor=0.5)
+    ref_y = 0.5 * (g.astype(np.float64) @ O.t2l_dense(w, d).T)
+    assert np.abs(np.asarray(y.data, np.float64) - ref_y).max() <= tol * max(1, np.abs(ref_y).max()) * 10
+
+
 def test_errors(gpu):
     tb = rc.TensorBasis(2, 2)
     a = rc.FreeTensor(np.zeros(7), tb)

@@ -450,3 +450,60 @@ def test_shuffle_adjoint_is_the_adjoint(rng):
     E = np.eye(size)
     brute = np.array([np.dot(arg, O.st_mul(op, E[i], w, d)) for i in range(size)])
     assert_allclose(adj, brute, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# CBH, piecewise abelian streams and Stream.simplify
+# ---------------------------------------------------------------------------
+
+def test_cbh_closed_forms(rng):
+    """cbh of one element is that element; of x and -x is 0; at width 2,
+    depth 2 (Hall basis e1, e2, [e1, e2]) cbh(a e1, b e2) = (a, b, ab/2), the
+    Baker-Campbell-Hausdorff series to second order; parallel level-one
+    elements add."""
+    w, d = 2, 2
+    x = rng.normal(size=3)
+    assert_array_equal(O.cbh([x], w, d), x)
+    assert_allclose(O.cbh([x, -x], w, d), 0, atol=1e-15)
+    a, b = 0.7, -1.3
+    assert_allclose(O.cbh([np.array([a, 0, 0.]), np.array([0, b, 0.])], w, d),
+                    [a, b, a * b / 2], atol=1e-15)
+    assert_allclose(O.cbh([np.array([a, 2 * a, 0.]), np.array([b, 2 * b, 0.])], w, d),
+                    [a + b, 2 * (a + b), 0], atol=1e-15)
+    assert_allclose(O.cbh([], w, d), 0, atol=0)
+
+
+def test_simplified_stream_keeps_the_signature(rng):
+    """Chen's identity: when every dyadic interval of a query holds whole
+    pieces, the simplified stream's signature over the support equals the
+    signature of the increments (unit pieces at resolution 0, and two pieces
+    inside the single dyadic interval [0, 8))."""
+    w, d, N = 3, 4, 8
+    inc = rng.normal(size=(N, w)) * 0.3
+    st = O.IncrementStream(inc, w, d)
+    full = O.signature(inc, w, d)
+    pas = O.simplify(st, [(float(i), i + 1.0) for i in range(N)], 0)
+    assert_allclose(pas.signature(0, N, 0), full, atol=1e-13)
+    pas2 = O.simplify(st, [(0.0, 3.0), (3.0, 8.0)], 0)
+    assert_allclose(pas2.signature(0, 8, 0), full, atol=1e-13)
+    # a sub-range that is a union of dyadic intervals at resolution 0
+    assert_allclose(pas.signature(2, 6, 0), O.signature(inc[2:6], w, d), atol=1e-13)
+
+
+def test_piecewise_abelian_partial_overlaps(rng):
+    """Known answers of the reference's multipliers for one piece [0, 1)
+    holding L: [0.5, 1) at resolution 1 gives (1 - 0.5) L; [0.25, 0.5) at
+    resolution 2 gives (1 - 0.25) L (the upper multiplier for an interior
+    interval); [0, 0.5) at resolution 1 gives (1 - 0) L (a piece sticking out
+    on the right takes the same branch); outside the support gives 0."""
+    w, d = 2, 3
+    L = rng.normal(size=5)
+    pas = O.PiecewiseAbelianStream([((0.0, 1.0), L)], w, d, 0, (0.0, 1.0))
+    assert_allclose(pas.log_signature(0.5, 1.0, 1), 0.5 * L, atol=1e-15)
+    assert_allclose(pas.log_signature(0.25, 0.5, 2), 0.75 * L, atol=1e-15)
+    assert_allclose(pas.log_signature(0.0, 0.5, 1), L, atol=1e-15)
+    assert_array_equal(pas.log_signature(2.0, 3.0, 1), np.zeros(5))
+    # two pieces in one dyadic interval combine by CBH
+    M = rng.normal(size=5)
+    pas2 = O.PiecewiseAbelianStream([((0.0, 0.5), L), ((0.5, 1.0), M)], w, d, 0, (0.0, 1.0))
+    assert_allclose(pas2.log_signature(0.0, 1.0, 0), O.cbh([L, M], w, d), atol=1e-15)
