@@ -416,6 +416,40 @@ def simplify_leg(torch, steps):
             "launch_tflops": tf, "launch_valu_frac": tf / PEAK_TFLOPS["f32"]}
 
 
+def serialize_leg(torch, inc, offs, lens, n_streams=256, levels=6, resolution=12):
+    """A dyadic cache of the first 256 streams of the batch (stored levels
+    0..6, float32) written to and read back from the binary cache format in
+    host memory: dump = device-to-host copy plus framing, load = parsing plus
+    host-to-device copy, set beside rebuilding the same cache on the GPU."""
+    from roughpy_amd import serialization as ser
+    from roughpy_amd import streams as rs
+    rows = int(offs[n_streams])
+    params = np.concatenate([np.arange(L, dtype=np.float64) / L for L in lens[:n_streams]])
+    cache = rs.DyadicSignatureCache(inc[:rows], params, offs[:n_streams + 1], W, D,
+                                    resolution=resolution, levels=levels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cache.build()
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    blob = ser.dumps(cache)
+    dump_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    back = ser.loads(blob)
+    torch.cuda.synchronize()
+    load_ms = (time.perf_counter() - t0) * 1e3
+    same = bool(torch.equal(back.cache, cache.cache))
+    gb = len(blob) / 1e9
+    del back, cache
+    torch.cuda.empty_cache()
+    return {"workload": f"dyadic cache of {n_streams} streams, width {W}, depth {D}, levels "
+                        f"0..{levels}, float32, in host memory",
+            "bytes": len(blob), "dump_ms": dump_ms, "dump_gb_s": gb / (dump_ms * 1e-3),
+            "load_ms": load_ms, "load_gb_s": gb / (load_ms * 1e-3), "rebuild_ms": build_ms,
+            "bit_identical": same}
+
+
 def graph_leg(torch, reps=200):
     """A launch-bound call: signatures of 16 short streams (width 5, depth 5,
     lengths 2..8), timed as eager calls and as replays of a HIP graph
@@ -490,10 +524,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,simplify,graph",
+    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,simplify,serialize,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, backward, gram, shuffle, simplify, graph; empty for "
-                         "none)")
+                         "(dyadic, fma, backward, gram, shuffle, simplify, serialize, graph; "
+                         "empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -606,6 +640,8 @@ def main():
         line["shuffle_w4_d6"] = shuffle_leg(torch, max(3, args.steps // 2), 1)
     if "simplify" in legs:
         line["simplify_w5_d5"] = simplify_leg(torch, max(3, args.steps // 2))
+    if "serialize" in legs:
+        line["serialize_cache"] = serialize_leg(torch, inc, offs, lens)
     if "graph" in legs:
         line["graph_replay"] = graph_leg(torch)
     if not args.no_cpu_baseline:

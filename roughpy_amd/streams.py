@@ -363,6 +363,49 @@ class DyadicSignatureCache:
         self.cache = _device.empty((n,), dt)
         self.build()
 
+    @classmethod
+    def _from_state(cls, increments, params, offsets, nodes, width, depth, resolution, levels,
+                    dtype):
+        """A cache from stored nodes, without rebuilding them."""
+        obj = cls.__new__(cls)
+        t = _device.torch()
+        obj.dtype = np.dtype(dtype)
+        obj.width, obj.depth = int(width), int(depth)
+        obj.resolution, obj.levels = int(resolution), int(levels)
+        obj._inc = _device.to_device(np.ascontiguousarray(increments), obj.dtype).reshape(
+            -1, obj.width)
+        obj._params = _device.to_device(np.ascontiguousarray(params, dtype=np.float64))
+        obj._offs = _device.to_device(np.ascontiguousarray(offsets, dtype=np.int64)).to(t.int64)
+        obj.n_streams = int(obj._offs.shape[0]) - 1
+        L = _capi.lib()
+        n = int(L.rpy_dyadic_cache_elements(obj.n_streams, obj.width, obj.depth, obj.levels))
+        if n != int(np.asarray(nodes).shape[0]):
+            raise ValueError("stored cache size does not match its shape")
+        obj.tsize = int(L.rpy_tensor_size(obj.width, obj.depth))
+        obj.cache = _device.to_device(np.ascontiguousarray(nodes), obj.dtype)
+        return obj
+
+    def dump(self, path) -> None:
+        """Writes the cache (nodes and the stream data they came from) to a
+        file; the counterpart of DyadicCachingLayer::dump_cache
+        (streams/src/dyadic_caching_layer.cpp:183-201)."""
+        from . import serialization
+        serialization.dump(self, path)
+
+    @classmethod
+    def load(cls, path) -> "DyadicSignatureCache":
+        """Reads a dumped cache back into device memory without rebuilding
+        it (DyadicCachingLayer::load_cache, dyadic_caching_layer.cpp:168-181)."""
+        from . import serialization
+        obj = serialization.load(path)
+        if not isinstance(obj, cls):
+            raise ValueError("file does not hold a dyadic signature cache")
+        return obj
+
+    def __reduce__(self):
+        from . import serialization
+        return (serialization.loads, (serialization.dumps(self),))
+
     def build(self):
         _capi.check(_capi.lib().rpy_dyadic_cache_build(
             self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
@@ -476,6 +519,13 @@ class LieIncrementStream:
             sup_support = (float(support[0]), float(support[1])) if not isinstance(
                 support, RealInterval) else (support.inf, support.sup)
         return cls(params, rows, width, depth, res, sup_support, dtype)
+
+    def __reduce__(self):
+        """Pickling stores the stream in the binary stream format, as the
+        reference pickles a stream's serialised archive
+        (roughpy/src/streams/stream.cpp:498-519)."""
+        from . import serialization
+        return (serialization.loads, (serialization.dumps(self),))
 
     # -- queries -------------------------------------------------------------
     def _parse(self, interval_or_inf, sup, resolution):
@@ -693,6 +743,10 @@ class PiecewiseAbelianStream:
         if int(self._lies.shape[1]) != self._lb.size():
             raise ValueError("piece data does not match the Lie basis size")
         del t
+
+    def __reduce__(self):
+        from . import serialization
+        return (serialization.loads, (serialization.dumps(self),))
 
     @property
     def pieces(self):
