@@ -620,10 +620,140 @@ int launch_inplace(void* lhs, const void* rhs, const DevBasis& b, Bounds lb, Bou
     return RPY_OK;
 }
 
+// ---------------------------------------------------------------------------
+// exp / fmexp / log for a compile-time shape.  The same truncated in-place
+// Horner as horner_kernel (products a <- a (x) x with x's unit dropped, levels
+// top-down, each output summed over lhs degrees high to low as
+// (a x) f), but every level offset and divisor is a constant and the thread
+// of a prefix u computes the W outputs u c together: for lhs degree ld the
+// lhs word is the first ld letters of u (one read) and the rhs words are
+// the rest of u followed by c (W contiguous reads).
+// ---------------------------------------------------------------------------
+
+template <typename T, int W, int D, int ml>
+RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* x, T f)
+{
+    chen::static_for<ml>([&](auto mi) {
+        constexpr int m = ml - int(decltype(mi)::value);  // level, top-down
+        constexpr uint32_t tasks = uint32_t(ipow(W, m - 1));
+        constexpr uint32_t dbm = uint32_t(fdb<W>(m));
+        for (uint32_t u = threadIdx.x; u < tasks; u += blockDim.x) {
+            T acc[W];
+            chen::static_for<W>([&](auto c) { acc[c] = T(0); });
+            chen::static_for<m>([&](auto li) {
+                constexpr int ld = m - 1 - int(decltype(li)::value);  // lhs degree, high to low
+                constexpr uint32_t k = uint32_t(ipow(W, m - 1 - ld));  // rhs letters from u
+                constexpr uint32_t dbl = uint32_t(fdb<W>(ld));
+                constexpr uint32_t dbr = uint32_t(fdb<W>(m - ld));
+                const uint32_t pi = u / k;
+                const uint32_t si = u - pi * k;
+                const T av = a[dbl + pi];
+                const T* xb = x + dbr + si * W;
+                chen::static_for<W>([&](auto c) { acc[c] += (av * xb[c]) * f; });
+            });
+            T* ao = a + dbm + u * W;
+            chen::static_for<W>([&](auto c) { ao[c] = acc[c]; });
+        }
+        __syncthreads();
+    });
+    if (threadIdx.x == 0) a[0] = T(0);
+    __syncthreads();
+}
+
+template <typename T, Horner H, int W, int D>
+__global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ out,
+                                                             const T* __restrict__ mul,
+                                                             const T* __restrict__ arg,
+                                                             int64_t batch, int64_t os, int64_t ms,
+                                                             int64_t as)
+{
+    constexpr uint32_t n = uint32_t(fdb<W>(D + 1));
+    constexpr uint32_t np = (n + 15) / 16 * 16;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* a = reinterpret_cast<T*>(smem);
+    T* xs = a + np;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        T* o = out + e * os;
+        const T* x = arg + e * as;
+        const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+            xs[i] = x[i];
+            if constexpr (H == Horner::FmExp)
+                a[i] = m[i];
+            else
+                a[i] = (H == Horner::Exp && i == 0) ? T(1) : T(0);
+        }
+        __syncthreads();
+        chen::static_for<D>([&](auto di) {
+            constexpr int deg = D - int(decltype(di)::value);
+            constexpr int ml = D - deg + 1;
+            if constexpr (H == Horner::Log) {
+                if (threadIdx.x == 0) {
+                    const T inv = T(1) / T(deg);
+                    a[0] = (deg % 2 == 0) ? a[0] - inv : a[0] + inv;
+                }
+                __syncthreads();
+                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1));
+            } else {
+                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1) / T(deg));
+                if constexpr (H == Horner::Exp) {
+                    if (threadIdx.x == 0) a[0] += T(1);
+                } else {
+                    for (uint32_t i = threadIdx.x; i < n; i += kBlock) a[i] += m[i];
+                }
+                __syncthreads();
+            }
+        });
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) o[i] = a[i];
+        __syncthreads();
+    }
+}
+
+template <typename T, Horner H, int W, int D>
+bool launch_horner_fixed(void* out, const void* mul, const void* arg, int64_t batch, int64_t os,
+                         int64_t ms, int64_t as, hipStream_t st)
+{
+    constexpr size_t np = (size_t(fdb<W>(D + 1)) + 15) / 16 * 16;
+    constexpr size_t lds = 2 * np * sizeof(T);
+    if constexpr (lds > kLdsBudget) {
+        return false;
+    } else {
+        hipLaunchKernelGGL((horner_fixed_kernel<T, H, W, D>), dim3(grid_for(batch)), dim3(kBlock),
+                           lds, st, (T*)out, (const T*)mul, (const T*)arg, batch, os, ms, as);
+        return true;
+    }
+}
+
+template <typename T, Horner H>
+using HornerFixed = bool (*)(void*, const void*, const void*, int64_t, int64_t, int64_t, int64_t,
+                             hipStream_t);
+
+template <typename T, Horner H>
+HornerFixed<T, H> find_horner_fixed(int w, int d)
+{
+#define RPY_F(W, D) \
+    if (w == W && d == D) return &launch_horner_fixed<T, H, W, D>;
+    RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
+    RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
+    RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
+    RPY_F(5, 2) RPY_F(5, 3) RPY_F(5, 4) RPY_F(5, 5)
+    RPY_F(6, 2) RPY_F(6, 3) RPY_F(6, 4) RPY_F(6, 5)
+    RPY_F(7, 2) RPY_F(7, 3) RPY_F(7, 4)
+    RPY_F(8, 2) RPY_F(8, 3) RPY_F(8, 4)
+#undef RPY_F
+    return nullptr;
+}
+
 template <typename T, Horner H>
 int launch_horner(void* out, const void* mul, const void* arg, const DevBasis& b,
                   int64_t batch, int64_t os, int64_t ms, int64_t as, hipStream_t st)
 {
+    if (auto f = find_horner_fixed<T, H>(b.width, b.depth)) {
+        if (f(out, mul, arg, batch, os, ms, as, st)) {
+            RPY_HIP_CHECK(hipGetLastError());
+            return RPY_OK;
+        }
+    }
     const Pow pw = make_pow(b.width, b.depth);
     const size_t lds = size_t(2) * size_t(b.size()) * sizeof(T);
     dim3 grid(grid_for(batch)), block(kBlock);

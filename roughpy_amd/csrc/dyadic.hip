@@ -226,26 +226,52 @@ constexpr int dbeg(int k)
 // fine pieces of one end are adjacent, so each end is one contiguous run of
 // rows.  Fills the row ranges of the runs and the index range of the stored
 // pieces; returns the piece count.
-RPY_FI int split_cover(int64_t A, int64_t B, int r, int J, const double* params, int64_t lo,
-                       int64_t hi, int64_t* pk, int* pn, int64_t* runs, int* coarse)
+// lower_bound over params[lo, hi) by one wave: each round the 64 lanes probe
+// 64 evenly spaced positions and a ballot keeps the segment holding the
+// first params >= v (params is sorted), so 4096 entries take two rounds of
+// loads instead of twelve dependent ones.  Every lane returns the result.
+RPY_FI int64_t lower_bound_wave(const double* params, int64_t lo, int64_t hi, double v)
+{
+    const int lane = int(threadIdx.x % 64);
+    int64_t L = lo, H = hi;  // the answer lies in [L, H]
+    while (H > L) {
+        const int64_t len = H - L;
+        if (len <= 64) {
+            const bool less = lane < len && params[L + lane] < v;
+            return L + int64_t(__popcll(__ballot(less)));
+        }
+        const int64_t step = (len + 63) / 64;
+        const int64_t pos = L + int64_t(lane) * step;
+        const bool less = pos < H && params[pos] < v;
+        const int k = __popcll(__ballot(less));  // probes below v form a prefix
+        const int64_t nl = k == 0 ? L : L + int64_t(k - 1) * step + 1;
+        const int64_t nh = (k == 64 || L + int64_t(k) * step >= H) ? H : L + int64_t(k) * step;
+        L = nl;
+        H = nh;
+    }
+    return L;
+}
+
+// The cover of [A, B) split into [fine run][stored nodes c0..c1)[fine run]:
+// the four run bounds are returned as parameter values (vals) for
+// lower_bound over the stream's params; need[i] says whether bound i is
+// searched (else it is the stream's first row).
+RPY_FI int plan_cover(int64_t A, int64_t B, int r, int J, int64_t* pk, int* pn, double* vals,
+                      int* need, int* coarse)
 {
     const int np = dyadic_cover(A, B, r, pk, pn);
     int c0 = 0;
     while (c0 < np && pn[c0] > J) ++c0;
     int c1 = c0;
     while (c1 < np && pn[c1] <= J) ++c1;
-    // c1 == np or every piece after c1 is fine
     auto start = [&](int p) { return std::ldexp(double(pk[p]), -pn[p]); };
     auto end = [&](int p) { return std::ldexp(double(pk[p] + 1), -pn[p]); };
-    runs[0] = runs[1] = runs[2] = runs[3] = lo;
-    if (c0 > 0) {
-        runs[0] = lower_bound_dev(params, lo, hi, start(0));
-        runs[1] = lower_bound_dev(params, lo, hi, end(c0 - 1));
-    }
-    if (c1 < np) {
-        runs[2] = lower_bound_dev(params, lo, hi, start(c1));
-        runs[3] = lower_bound_dev(params, lo, hi, end(np - 1));
-    }
+    need[0] = need[1] = c0 > 0;
+    need[2] = need[3] = c1 < np;
+    vals[0] = c0 > 0 ? start(0) : 0.0;
+    vals[1] = c0 > 0 ? end(c0 - 1) : 0.0;
+    vals[2] = c1 < np ? start(c1) : 0.0;
+    vals[3] = c1 < np ? end(np - 1) : 0.0;
     coarse[0] = c0;
     coarse[1] = c1;
     return np;
@@ -258,26 +284,29 @@ RPY_FI int split_cover(int64_t A, int64_t B, int r, int J, const double* params,
 template <typename T, int W, int D>
 RPY_FI void wg_mul_inplace_t(T* S, const T* N)
 {
+    // the thread of a prefix u computes the W outputs u c of level m: for
+    // rhs degree i >= 1 the lhs word is a prefix of u (one read) and the rhs
+    // words are the rest of u followed by c (W contiguous reads)
     chen::static_for<D>([&](auto i0) {
         constexpr int m = D - int(decltype(i0)::value);
-        constexpr int n = int(ipow(W, m));
+        constexpr int tasks = int(ipow(W, m - 1));
         constexpr int dbm = dbeg<W>(m);
-        for (int idx = threadIdx.x; idx < n; idx += kBlock) {
-            T lv[m + 1], rv[m + 1];
-            chen::static_for<m + 1>([&](auto k0) {
-                constexpr int i = int(decltype(k0)::value);
-                constexpr int wi = int(ipow(W, i));
-                const int pre = idx / wi;
-                const int suf = idx - pre * wi;
-                lv[i] = S[dbeg<W>(m - i) + pre];
-                rv[i] = N[dbeg<W>(i) + suf];
+        for (int u = threadIdx.x; u < tasks; u += kBlock) {
+            T acc[W];
+            const T n0 = N[0];
+            chen::static_for<W>([&](auto c) { acc[c] = S[dbm + u * W + int(decltype(c)::value)] * n0; });
+            chen::static_for<m>([&](auto k0) {
+                constexpr int i = int(decltype(k0)::value) + 1;  // rhs degree
+                constexpr int wi = int(ipow(W, i - 1));
+                constexpr int dbl = dbeg<W>(m - i);
+                constexpr int dbr = dbeg<W>(i);
+                const int pre = u / wi;
+                const int suf = u - pre * wi;
+                const T lv = S[dbl + pre];
+                const T* rb = N + dbr + suf * W;
+                chen::static_for<W>([&](auto c) { acc[c] += lv * rb[int(decltype(c)::value)]; });
             });
-            T acc = T(0);
-            chen::static_for<m + 1>([&](auto k0) {
-                constexpr int i = int(decltype(k0)::value);
-                acc += lv[i] * rv[i];
-            });
-            S[dbm + idx] = acc;
+            chen::static_for<W>([&](auto c) { S[dbm + u * W + int(decltype(c)::value)] = acc[c]; });
         }
         __syncthreads();
     });
@@ -314,29 +343,49 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
     __shared__ int64_t runs[4];
     __shared__ int coarse[2];
     __shared__ int64_t qs;
+    __shared__ int64_t qlo, qhi;
+    __shared__ double bvals[4];
+    __shared__ int bneed[4];
     __shared__ T XT[2 * kQueryChunk * kDW];
     const int half = threadIdx.x / 128;   // 0: left run, 1: right run
     const int q_lane = threadIdx.x % 128;
+    static_assert(kBlock / 64 >= 4, "one wave per run bound");
 
     for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
         if (threadIdx.x == 0) {
             const int64_t s = q_stream[q];
             const bool valid = s >= 0 && s < n_streams;
             int c[2] = {0, 0};
-            int64_t rr[4] = {0, 0, 0, 0};
+            int need[4] = {0, 0, 0, 0};
+            double vals[4] = {0.0, 0.0, 0.0, 0.0};
+            int64_t lo = 0, hi = 0;
             if (valid) {
+                lo = offsets[s];
+                hi = offsets[s + 1];
                 const double scale = std::ldexp(1.0, r);
                 double a = std::floor(q_inf[q] * scale), e = std::floor(q_sup[q] * scale);
                 a = std::min(std::max(a, 0.0), scale);
                 e = std::min(std::max(e, 0.0), scale);
-                if (a < e)
-                    split_cover(int64_t(a), int64_t(e), r, J, params, offsets[s], offsets[s + 1],
-                                pk, pn, rr, c);
+                if (a < e) plan_cover(int64_t(a), int64_t(e), r, J, pk, pn, vals, need, c);
             }
-            for (int i = 0; i < 4; ++i) runs[i] = rr[i];
+            for (int i = 0; i < 4; ++i) {
+                bvals[i] = vals[i];
+                bneed[i] = need[i];
+            }
+            qlo = lo;
+            qhi = hi;
             coarse[0] = c[0];
             coarse[1] = c[1];
             qs = valid ? s : 0;
+        }
+        __syncthreads();
+        {
+            // the run bounds: wave i searches bound i
+            const int wv = int(threadIdx.x / 64);
+            if (wv < 4) {
+                const int64_t b = bneed[wv] ? lower_bound_wave(params, qlo, qhi, bvals[wv]) : qlo;
+                if (threadIdx.x % 64 == 0) runs[wv] = b;
+            }
         }
         __syncthreads();
         const int64_t s = qs;
