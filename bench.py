@@ -224,6 +224,59 @@ def fma_leg(torch, steps, warmup):
     return res
 
 
+def dense_leg(torch, steps, warmup):
+    """The other dense operations of the path on a batch of 16384 tensors of
+    width 5, depth 5 (3906 coefficients; Lie dimension 829), inputs resident
+    in HBM, through the Python mirror of roughpy.compute: in-place product,
+    exp, fmexp, log, antipode, adjoint of left multiplication, lie_to_tensor
+    and tensor_to_lie.  Each is HBM-bound; the algorithmic bytes per element
+    are the operands read plus the result written."""
+    from math import factorial
+
+    from roughpy_amd import compute as rc
+    from roughpy_amd.bases import LieBasis, TensorBasis
+    w, d, batch = 5, 5, 16384
+    tb, lb = TensorBasis(w, d), LieBasis(w, d)
+    n, nl = tb.size(), lb.size()
+    db = tb.degree_begin
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 80)
+    scale = torch.empty(n, device="cuda")
+    for k in range(d + 1):
+        scale[db[k]:db[k + 1]] = 1.0 / factorial(k)
+    res = {"workload": f"batch {batch}, width {w}, depth {d} ({n} coefficients, Lie dimension {nl})"}
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        A = ((torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale).to(dt)
+        B = ((torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale).to(dt)
+        A[:, 0] = 1
+        B[:, 0] = 1
+        X = B.clone()
+        X[:, 0] = 0
+        Lz = (torch.rand((batch, nl), generator=gen, device="cuda") * 2 - 1).to(dt)
+        es = A.element_size()
+        ops = {
+            "inplace_mul": (lambda: rc.ft_inplace_mul(rc.FreeTensor(A, tb), rc.FreeTensor(B, tb)),
+                            3 * n),
+            "exp": (lambda: rc.ft_exp(rc.FreeTensor(X, tb)), 2 * n),
+            "fmexp": (lambda: rc.ft_fmexp(rc.FreeTensor(B, tb), rc.FreeTensor(X, tb)), 3 * n),
+            "log": (lambda: rc.ft_log(rc.FreeTensor(B, tb)), 2 * n),
+            "antipode": (lambda: rc.antipode(rc.FreeTensor(B, tb)), 2 * n),
+            "adjoint_left_mul": (lambda: rc.ft_adjoint_left_mul(rc.FreeTensor(B, tb),
+                                                                 rc.ShuffleTensor(X, tb)), 3 * n),
+            "lie_to_tensor": (lambda: rc.lie_to_tensor(rc.Lie(Lz, lb), tb), nl + n),
+            "tensor_to_lie": (lambda: rc.tensor_to_lie(rc.FreeTensor(B, tb), lb), n + nl),
+        }
+        out = {}
+        for op, (fn, vals) in ops.items():
+            ms = time_kernel(fn, steps, warmup, torch)
+            gbs = float(vals) * es * batch / (ms * 1e-3) / 1e9
+            out[op] = {"ms": ms, "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+        res[name] = out
+        del A, B, X, Lz
+    torch.cuda.empty_cache()
+    return res
+
+
 def backward_leg(torch, steps, warmup):
     """Signature forward + reverse-Chen backward, width 8, depth 4 (4681
     coefficients), 4096 streams of length 256, N(0, 1/256) increments, loss
@@ -524,10 +577,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="dyadic,fma,backward,gram,shuffle,simplify,serialize,graph",
+    ap.add_argument("--legs",
+                    default="dyadic,fma,dense,backward,gram,shuffle,simplify,serialize,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, backward, gram, shuffle, simplify, serialize, graph; "
-                         "empty for none)")
+                         "(dyadic, fma, dense, backward, gram, shuffle, simplify, serialize, "
+                         "graph; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -632,6 +686,8 @@ def main():
     torch.cuda.empty_cache()
     if "fma" in legs:
         line["fma_w4_d6"] = fma_leg(torch, max(3, args.steps // 2), 1)
+    if "dense" in legs:
+        line["dense_w5_d5"] = dense_leg(torch, max(3, args.steps // 2), 1)
     if "backward" in legs:
         line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
     if "gram" in legs:

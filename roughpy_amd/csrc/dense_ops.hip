@@ -413,6 +413,179 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
     }
 }
 
+// LDS index with bits 5..9 folded into bits 0..4: a 32-aligned block stays in
+// place and power-of-two strides spread over the banks.
+RPY_FI uint32_t swz(uint32_t j) { return j ^ ((j >> 5) & 31u); }
+
+// antipode with the tensor staged in LDS: coalesced read, then each output
+// word gathers its reversed word from LDS and is written coalesced (and the
+// call may be in place).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void antipode_lds_kernel(T* __restrict__ out,
+                                                             const T* __restrict__ arg,
+                                                             DevBasis b, Pow pw, int64_t batch,
+                                                             int64_t os, int64_t as)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sx = reinterpret_cast<T*>(smem);
+    const uint32_t n = uint32_t(b.size());
+    const uint32_t w = uint32_t(b.width);
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        const T* a = arg + e * as;
+        __syncthreads();  // the previous tensor's readers are done
+        for (uint32_t j = threadIdx.x; j < n; j += kBlock) sx[swz(j)] = a[j];
+        __syncthreads();
+        T* o = out + e * os;
+        for (int d = 0; d <= b.depth; ++d) {
+            const uint32_t base = uint32_t(b.db[d]);
+            const uint32_t sz = uint32_t(b.level_size(d));
+            for (uint32_t i = threadIdx.x; i < sz; i += kBlock) {
+                uint32_t x = i, rev = 0;
+                for (int k = 0; k < d; ++k) {
+                    const uint32_t q = fdiv(x, pw.pw[1]);
+                    rev = rev * w + (x - q * w);
+                    x = q;
+                }
+                const T v = sx[swz(base + rev)];
+                o[base + i] = (d & 1) ? -v : v;
+            }
+        }
+    }
+}
+
+// Work split of the adjoint of left multiplication,
+// out(v) += sum_{u} op(u) arg(u v): output level od sums db[D - od + 1]
+// terms per word, so the words of low levels are split over groups of
+// g[od] threads (a power of two <= 64, aligned inside a wave) whose partial
+// sums meet in a fixed butterfly; vbase[od] is the level's first virtual
+// thread.
+// Levels whose words need no group (g = 1) give each virtual thread the W
+// words q c, c < W, when W <= kAdjTile (tile[od] = W): the terms of a tile
+// share their op value and read W contiguous arg values.
+constexpr int kAdjTile = 8;
+
+struct AdjLPlan {
+    int g[kMaxDepth + 1];
+    int tile[kMaxDepth + 1];
+    int vbase[kMaxDepth + 2];
+};
+
+inline AdjLPlan make_adjl_plan(const DevBasis& b)
+{
+    AdjLPlan p{};
+    int64_t v = 0;
+    for (int od = 0; od <= b.depth; ++od) {
+        const int64_t terms = b.db[b.depth - od + 1];
+        int g = 1;
+        while (g < 64 && terms / (2 * g) >= 16) g *= 2;
+        p.g[od] = g;
+        p.tile[od] = (g == 1 && od >= 1 && b.width <= kAdjTile) ? b.width : 1;
+        p.vbase[od] = int(v);
+        v += int64_t(g) * b.level_size(od) / p.tile[od];
+    }
+    p.vbase[b.depth + 1] = int(v);
+    return p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void adj_lmul_lds_kernel(T* __restrict__ out,
+                                                             const T* __restrict__ op,
+                                                             const T* __restrict__ arg,
+                                                             DevBasis b, Pow pw, AdjLPlan plan,
+                                                             int64_t batch, int64_t os,
+                                                             int64_t ps, int64_t as)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t n = uint32_t(b.size());
+    const uint32_t np = (n + 3) / 4 * 4;
+    T* P = reinterpret_cast<T*>(smem);
+    T* A = P + np;
+    const int D = b.depth;
+    const int V = plan.vbase[D + 1];
+    const int Vr = (V + kBlock - 1) / kBlock * kBlock;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        const T* p = op + e * ps;
+        const T* a = arg + e * as;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
+            P[j] = p[j];
+            A[j] = a[j];
+        }
+        __syncthreads();
+        T* o = out + e * os;
+        for (int vt = threadIdx.x; vt < Vr; vt += kBlock) {
+            // every lane of a wave runs the same iterations; lanes past V
+            // hold zero and write nothing
+            int od = 0;
+            while (od < D && vt >= plan.vbase[od + 1]) ++od;
+            const bool live = vt < V;
+            const int g = live ? plan.g[od] : 1;
+            const int r = live ? vt - plan.vbase[od] : 0;
+            if (live && plan.tile[od] > 1) {
+                // W words q c of level od, every term of each
+                const uint32_t wod = pw.pw[od].d;
+                const uint32_t q = uint32_t(r);
+                const uint32_t W = uint32_t(b.width);
+                T accs[kAdjTile];
+#pragma unroll
+                for (int c = 0; c < kAdjTile; ++c) accs[c] = T(0);
+                for (int pd = D - od; pd >= 0; --pd) {
+                    const uint32_t psz = uint32_t(b.level_size(pd));
+                    const T* Pp = P + b.db[pd];
+                    const T* Ap = A + b.db[od + pd] + q * W;
+                    for (uint32_t u = 0; u < psz; ++u) {
+                        const T pv = Pp[u];
+                        const T* ar = Ap + int64_t(u) * wod;
+#pragma unroll
+                        for (int c = 0; c < kAdjTile; ++c)
+                            if (uint32_t(c) < W) accs[c] += pv * ar[c];
+                    }
+                }
+                T* oo = o + b.db[od] + q * W;
+#pragma unroll
+                for (int c = 0; c < kAdjTile; ++c)
+                    if (uint32_t(c) < W) oo[c] += accs[c];
+            }
+            const bool grouped = live && plan.tile[od] == 1;
+            const uint32_t vw = uint32_t(r / g);        // output word of level od
+            const int k = r - int(vw) * g;               // member of its group
+            T acc = T(0);
+            if (grouped) {
+                const uint32_t wod = pw.pw[od].d;       // W^od
+                const int64_t terms = b.db[D - od + 1];
+                const int64_t chunk = (terms + g - 1) / g;
+                int64_t f = int64_t(k) * chunk;
+                int64_t fe = f + chunk < terms ? f + chunk : terms;
+                // flat term f -> (pd, u), pd descending from D - od
+                int pd = D - od;
+                while (pd > 0 && f >= b.level_size(pd)) {
+                    f -= b.level_size(pd);
+                    fe -= b.level_size(pd);
+                    --pd;
+                }
+                uint32_t u = uint32_t(f);
+                for (int64_t t = fe - f; t > 0; --t) {
+                    acc += P[b.db[pd] + u] * A[b.db[od + pd] + int64_t(u) * wod + vw];
+                    if (++u == uint32_t(b.level_size(pd)) && pd > 0) {
+                        --pd;
+                        u = 0;
+                    }
+                }
+            }
+            // group butterfly (groups are aligned powers of two inside a wave),
+            // skipped by waves with no group
+            const int gw = grouped ? g : 1;
+            if (__any(gw > 1)) {
+                for (int off = 1; off < 64; off *= 2) {
+                    const T other = __shfl_xor(acc, off, 64);
+                    if (off < gw) acc += other;
+                }
+            }
+            if (grouped && k == 0) o[b.db[od] + vw] += acc;
+        }
+    }
+}
+
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int check_bounds(const DevBasis& b, rpy_degree_bounds in, Bounds& out, const char* name)
@@ -850,6 +1023,19 @@ int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_ten
     if (batch == 0) return RPY_OK;
     if (!out || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     const Pow pw = make_pow(b.width, 1);
+    const size_t es = dtype == RPY_F64 ? 8 : 4;
+    const size_t lds = size_t((b.size() + 31) / 32 * 32) * es;
+    if (lds <= kLdsBudget) {
+        dim3 grid(grid_for(batch)), block(kBlock);
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL(antipode_lds_kernel<double>, grid, block, lds, as_stream(stream),
+                               (double*)out, (const double*)arg, b, pw, batch, os, as);
+        else
+            hipLaunchKernelGGL(antipode_lds_kernel<float>, grid, block, lds, as_stream(stream),
+                               (float*)out, (const float*)arg, b, pw, batch, os, as);
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
     dim3 grid(blocks_elementwise(batch * b.size())), block(kBlock);
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(antipode_kernel<double>, grid, block, 0, as_stream(stream),
@@ -872,6 +1058,22 @@ int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dt
     if (batch == 0) return RPY_OK;
     if (!out || !op || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     dim3 grid(grid_for(batch)), block(kBlock);
+    const size_t es = dtype == RPY_F64 ? 8 : 4;
+    const size_t lds = 2 * size_t((b.size() + 3) / 4 * 4) * es;
+    if (lds <= kLdsBudget && b.depth <= kMaxDepth) {
+        const Pow pw = make_pow(b.width, b.depth);
+        const AdjLPlan plan = make_adjl_plan(b);
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL(adj_lmul_lds_kernel<double>, grid, block, lds, as_stream(stream),
+                               (double*)out, (const double*)op, (const double*)arg, b, pw, plan,
+                               batch, os, ps, as);
+        else
+            hipLaunchKernelGGL(adj_lmul_lds_kernel<float>, grid, block, lds, as_stream(stream),
+                               (float*)out, (const float*)op, (const float*)arg, b, pw, plan,
+                               batch, os, ps, as);
+        RPY_HIP_CHECK(hipGetLastError());
+        return RPY_OK;
+    }
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(adj_lmul_kernel<double>, grid, block, 0, as_stream(stream),
                            (double*)out, (const double*)op, (const double*)arg, b, batch, os, ps,

@@ -115,6 +115,9 @@ def test_dense_ops_batched_vs_oracle(gpu, rng, w, d, dtype):
         _close(rc.ft_fmexp(rc.FreeTensor(A, tb), rc.FreeTensor(X, tb)).data,
                O.ft_fmexp(a64, x64, w, d), w, d, dtype)
     assert_array_equal(rc.antipode(rc.FreeTensor(C, tb)).data, O.antipode(C, w, d))
+    # adjoint of left multiplication (op A, argument C) against the oracle
+    adj = rc.ft_adjoint_left_mul(rc.FreeTensor(A, tb), rc.ShuffleTensor(C.copy(), tb)).data
+    _close(adj, O.ft_adjoint_left_mul(a64, c64, w, d), w, d, dtype)
 
 
 def test_degree_bounds_and_torch_inputs(gpu, rng):
