@@ -803,11 +803,17 @@ int launch_inplace(void* lhs, const void* rhs, const DevBasis& b, Bounds lb, Bou
 // the rest of u followed by c (W contiguous reads).
 // ---------------------------------------------------------------------------
 
+// a <- a (x) x with x's unit dropped, levels ml..1 top-down.  a and xs hold
+// levels 0..D-1 in LDS; when ml == D the top level's one x term (lhs degree
+// 0) reads the global xg and the top level's outputs go straight to the
+// global o (plus m when given): no product ever reads a's top level.
 template <typename T, int W, int D, int ml>
-RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* x, T f)
+RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* xs, T f, const T* __restrict__ xg,
+                                   T* __restrict__ o, const T* __restrict__ mg)
 {
     chen::static_for<ml>([&](auto mi) {
         constexpr int m = ml - int(decltype(mi)::value);  // level, top-down
+        constexpr bool top = m == D;
         constexpr uint32_t tasks = uint32_t(ipow(W, m - 1));
         constexpr uint32_t dbm = uint32_t(fdb<W>(m));
         for (uint32_t u = threadIdx.x; u < tasks; u += blockDim.x) {
@@ -821,11 +827,19 @@ RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* x, T f)
                 const uint32_t pi = u / k;
                 const uint32_t si = u - pi * k;
                 const T av = a[dbl + pi];
-                const T* xb = x + dbr + si * W;
+                const T* xb = (top && ld == 0 ? xg : xs) + dbr + si * W;
                 chen::static_for<W>([&](auto c) { acc[c] += (av * xb[c]) * f; });
             });
-            T* ao = a + dbm + u * W;
-            chen::static_for<W>([&](auto c) { ao[c] = acc[c]; });
+            if constexpr (top) {
+                T* oo = o + dbm + u * W;
+                if (mg)
+                    chen::static_for<W>([&](auto c) { oo[c] = acc[c] + mg[dbm + u * W + c]; });
+                else
+                    chen::static_for<W>([&](auto c) { oo[c] = acc[c]; });
+            } else {
+                T* ao = a + dbm + u * W;
+                chen::static_for<W>([&](auto c) { ao[c] = acc[c]; });
+            }
         }
         __syncthreads();
     });
@@ -840,8 +854,10 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
                                                              int64_t batch, int64_t os, int64_t ms,
                                                              int64_t as)
 {
-    constexpr uint32_t n = uint32_t(fdb<W>(D + 1));
-    constexpr uint32_t np = (n + 15) / 16 * 16;
+    // levels 0..D-1 of the accumulator and of x live in LDS; only the last
+    // product writes the top level, and it writes it to out directly
+    constexpr uint32_t ns = uint32_t(fdb<W>(D));
+    constexpr uint32_t np = (ns + 15) / 16 * 16;
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* xs = a + np;
@@ -849,7 +865,7 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
         T* o = out + e * os;
         const T* x = arg + e * as;
         const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (uint32_t i = threadIdx.x; i < ns; i += kBlock) {
             xs[i] = x[i];
             if constexpr (H == Horner::FmExp)
                 a[i] = m[i];
@@ -866,18 +882,19 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
                     a[0] = (deg % 2 == 0) ? a[0] - inv : a[0] + inv;
                 }
                 __syncthreads();
-                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1));
+                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1), x, o, nullptr);
             } else {
-                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1) / T(deg));
+                wg_mul_by_nounit_fixed<T, W, D, ml>(a, xs, T(1) / T(deg), x, o,
+                                                    H == Horner::FmExp ? m : nullptr);
                 if constexpr (H == Horner::Exp) {
                     if (threadIdx.x == 0) a[0] += T(1);
                 } else {
-                    for (uint32_t i = threadIdx.x; i < n; i += kBlock) a[i] += m[i];
+                    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) a[i] += m[i];
                 }
                 __syncthreads();
             }
         });
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) o[i] = a[i];
+        for (uint32_t i = threadIdx.x; i < ns; i += kBlock) o[i] = a[i];
         __syncthreads();
     }
 }
@@ -886,7 +903,7 @@ template <typename T, Horner H, int W, int D>
 bool launch_horner_fixed(void* out, const void* mul, const void* arg, int64_t batch, int64_t os,
                          int64_t ms, int64_t as, hipStream_t st)
 {
-    constexpr size_t np = (size_t(fdb<W>(D + 1)) + 15) / 16 * 16;
+    constexpr size_t np = (size_t(fdb<W>(D)) + 15) / 16 * 16;
     constexpr size_t lds = 2 * np * sizeof(T);
     if constexpr (lds > kLdsBudget) {
         return false;
