@@ -400,7 +400,23 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
 #pragma unroll
             for (int q = 0; q < E; ++q) acc[q] = T(0);
             const int64_t pe = offs[r + 1];
-            for (int64_t p = offs[r]; p < pe; ++p) {
+            int64_t p = offs[r];
+            // entries in groups of 4: the group's index and value loads are
+            // issued together, then applied in order
+            for (; p + 4 <= pe; p += 4) {
+                int64_t c[4];
+                T v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    c[j] = idx[p + j];
+                    v[j] = val[p + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += (v[j] * s[q * cols + c[j]]) * scale;
+            }
+            for (; p < pe; ++p) {
                 const int64_t c = idx[p];
                 const T v = val[p];
 #pragma unroll
