@@ -658,9 +658,12 @@ RPY_FI Acc fma_output(Acc acc, Acc top_l, Acc top_r, int idx, const Acc* sl, con
     return a + sl[0] * top_r;
 }
 
-template <typename T, typename Acc, int W, int D>
-__global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* __restrict__ out,
-                                                             const T* __restrict__ lhs,
+// Accum = false computes out = lhs (x) rhs without reading out; with out equal
+// to lhs that is the in-place product: each output word is written by the
+// thread that read its own lhs top-level value, lower lhs levels come from the
+// LDS copy, and the unit term reads lhs[0] before out[0] is written.
+template <typename T, typename Acc, int W, int D, bool Accum = true>
+__global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* out, const T* lhs,
                                                              const T* __restrict__ rhs,
                                                              int64_t batch, int64_t os,
                                                              int64_t ls, int64_t rs)
@@ -684,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* __restrict__ ou
             for (int q = 0; q < kTopU; ++q) {
                 const int idx = base + q * kBlock;
                 const bool ok = idx < kN;
-                t_o[q] = ok ? o[dbD + idx] : T(0);
+                t_o[q] = (ok && Accum) ? o[dbD + idx] : T(0);
                 t_l[q] = ok ? l[dbD + idx] : T(0);
                 t_r[q] = ok ? r[dbD + idx] : T(0);
             }
@@ -716,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* __restrict__ ou
 #pragma unroll
                 for (int q = 0; q < kU; ++q) {
                     const int idx = base + q * kBlock;
-                    acc[q] = idx < n ? Acc(o[dbm + idx]) : Acc(0);
+                    acc[q] = (idx < n && Accum) ? Acc(o[dbm + idx]) : Acc(0);
                 }
 #pragma unroll
                 for (int q = 0; q < kU; ++q) {
@@ -727,7 +730,10 @@ __global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* __restrict__ ou
                 }
             }
         });
-        if (threadIdx.x == 0) o[0] = T(Acc(o[0]) + Acc(l[0]) * Acc(r[0]));
+        if (threadIdx.x == 0) {
+            const Acc prod = Acc(l[0]) * Acc(r[0]);
+            o[0] = Accum ? T(Acc(o[0]) + prod) : T(prod);
+        }
         __syncthreads();
     }
 }
@@ -743,6 +749,34 @@ void launch_fma_fixed(void* out, const void* lhs, const void* rhs, int64_t batch
 
 using FmaFixed = void (*)(void*, const void*, const void*, int64_t, int64_t, int64_t, int64_t,
                           hipStream_t);
+
+// in place: lhs <- lhs (x) rhs at full depth
+template <typename T, int W, int D>
+void launch_inplace_fixed(void* lhs, const void* rhs, int64_t batch, int64_t ls, int64_t rs,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL((ft_fma_fixed_kernel<T, T, W, D, false>), dim3(grid_for(batch)),
+                       dim3(kBlock), 0, st, (T*)lhs, (const T*)lhs, (const T*)rhs, batch, ls, ls,
+                       rs);
+}
+
+using InplaceFixed = void (*)(void*, const void*, int64_t, int64_t, int64_t, hipStream_t);
+
+template <typename T>
+InplaceFixed find_inplace_fixed(int w, int d)
+{
+#define RPY_F(W, D) \
+    if (w == W && d == D) return &launch_inplace_fixed<T, W, D>;
+    RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
+    RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
+    RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
+    RPY_F(5, 2) RPY_F(5, 3) RPY_F(5, 4) RPY_F(5, 5)
+    RPY_F(6, 2) RPY_F(6, 3) RPY_F(6, 4) RPY_F(6, 5)
+    RPY_F(7, 2) RPY_F(7, 3) RPY_F(7, 4)
+    RPY_F(8, 2) RPY_F(8, 3) RPY_F(8, 4)
+#undef RPY_F
+    return nullptr;
+}
 
 template <typename T, typename Acc>
 FmaFixed find_fma_fixed(int w, int d)
@@ -1040,6 +1074,15 @@ int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
     if ((rc = check_batch(batch, {ls, rs}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!lhs || !rhs) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    if (lb.lo == 0 && rb.lo == 0 && lb.hi == b.depth && rb.hi == b.depth && lhs != rhs) {
+        const InplaceFixed f = dtype == RPY_F64 ? find_inplace_fixed<double>(b.width, b.depth)
+                                                : find_inplace_fixed<float>(b.width, b.depth);
+        if (f) {
+            f(lhs, rhs, batch, ls, rs, as_stream(stream));
+            RPY_HIP_CHECK(hipGetLastError());
+            return RPY_OK;
+        }
+    }
     return dtype == RPY_F64 ? launch_inplace<double>(lhs, rhs, b, lb, rb, batch, ls, rs,
                                                      as_stream(stream))
                             : launch_inplace<float>(lhs, rhs, b, lb, rb, batch, ls, rs,
