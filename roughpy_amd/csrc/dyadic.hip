@@ -226,57 +226,6 @@ constexpr int dbeg(int k)
 // fine pieces of one end are adjacent, so each end is one contiguous run of
 // rows.  Fills the row ranges of the runs and the index range of the stored
 // pieces; returns the piece count.
-// lower_bound over params[lo, hi) by one wave: each round the 64 lanes probe
-// 64 evenly spaced positions and a ballot keeps the segment holding the
-// first params >= v (params is sorted), so 4096 entries take two rounds of
-// loads instead of twelve dependent ones.  Every lane returns the result.
-RPY_FI int64_t lower_bound_wave(const double* params, int64_t lo, int64_t hi, double v)
-{
-    const int lane = int(threadIdx.x % 64);
-    int64_t L = lo, H = hi;  // the answer lies in [L, H]
-    while (H > L) {
-        const int64_t len = H - L;
-        if (len <= 64) {
-            const bool less = lane < len && params[L + lane] < v;
-            return L + int64_t(__popcll(__ballot(less)));
-        }
-        const int64_t step = (len + 63) / 64;
-        const int64_t pos = L + int64_t(lane) * step;
-        const bool less = pos < H && params[pos] < v;
-        const int k = __popcll(__ballot(less));  // probes below v form a prefix
-        const int64_t nl = k == 0 ? L : L + int64_t(k - 1) * step + 1;
-        const int64_t nh = (k == 64 || L + int64_t(k) * step >= H) ? H : L + int64_t(k) * step;
-        L = nl;
-        H = nh;
-    }
-    return L;
-}
-
-// The cover of [A, B) split into [fine run][stored nodes c0..c1)[fine run]:
-// the four run bounds are returned as parameter values (vals) for
-// lower_bound over the stream's params; need[i] says whether bound i is
-// searched (else it is the stream's first row).
-RPY_FI int plan_cover(int64_t A, int64_t B, int r, int J, int64_t* pk, int* pn, double* vals,
-                      int* need, int* coarse)
-{
-    const int np = dyadic_cover(A, B, r, pk, pn);
-    int c0 = 0;
-    while (c0 < np && pn[c0] > J) ++c0;
-    int c1 = c0;
-    while (c1 < np && pn[c1] <= J) ++c1;
-    auto start = [&](int p) { return std::ldexp(double(pk[p]), -pn[p]); };
-    auto end = [&](int p) { return std::ldexp(double(pk[p] + 1), -pn[p]); };
-    need[0] = need[1] = c0 > 0;
-    need[2] = need[3] = c1 < np;
-    vals[0] = c0 > 0 ? start(0) : 0.0;
-    vals[1] = c0 > 0 ? end(c0 - 1) : 0.0;
-    vals[2] = c1 < np ? start(c1) : 0.0;
-    vals[3] = c1 < np ? end(np - 1) : 0.0;
-    coarse[0] = c0;
-    coarse[1] = c1;
-    return np;
-}
-
 // Unrolled S <- S (x) N for a compile-time shape, both tensors in LDS: for
 // each output level (top-down, barrier per level) every thread takes outputs
 // idx = tid + 256 q and sums lhs degree m-i times rhs degree i, i = 0..m
@@ -322,12 +271,87 @@ RPY_FI void wg_mul_inplace_t(T* S, const T* N)
 //    signature in registers (chen_core.hpp), then write it to S and R;
 // 3. S is multiplied by each stored node (the next node's coefficients are
 //    loaded into registers while the current product runs) and by R.
+// Query plans, one thread per query, computed ahead of the query kernel so
+// that a query's workgroup starts from one load.  The cover of [A, B) is
+// [fine run][stored nodes][fine run] (piece sizes grow, then shrink), so one
+// greedy pass over it yields the run bounds and the stored nodes' cache
+// offsets.  Record (int64 words): stream (-1 if invalid), the four run row
+// bounds, the node count, then the node offsets (at most 2 (J + 1)).
+constexpr int kPlanHead = 6;
+
+inline int plan_record_words(int J) { return kPlanHead + 2 * (J + 1); }
+
+__global__ __launch_bounds__(256) void dyadic_query_plan_kernel(
+        int64_t* __restrict__ plans, int rec, const double* __restrict__ params,
+        const int64_t* __restrict__ offsets, int64_t n_streams, int r, int J,
+        const int64_t* __restrict__ q_stream, const double* __restrict__ q_inf,
+        const double* __restrict__ q_sup, int64_t n_queries)
+{
+    const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (q >= n_queries) return;
+    int64_t* P = plans + q * rec;
+    const int64_t s = q_stream[q];
+    const bool valid = s >= 0 && s < n_streams;
+    int64_t runs[4] = {0, 0, 0, 0};
+    int64_t cnt = 0;
+    if (valid) {
+        const int64_t lo = offsets[s], hi = offsets[s + 1];
+        runs[0] = runs[1] = runs[2] = runs[3] = lo;
+        const double scale = std::ldexp(1.0, r);
+        double a = std::floor(q_inf[q] * scale), e = std::floor(q_sup[q] * scale);
+        a = std::min(std::max(a, 0.0), scale);
+        e = std::min(std::max(e, 0.0), scale);
+        if (a < e) {
+            int64_t A = int64_t(a);
+            const int64_t B = int64_t(e);
+            bool stored = false, left = false, right = false;
+            double l0 = 0.0, l1 = 0.0, r0 = 0.0, r1 = 0.0;
+            while (A < B) {
+                int lg = r;
+                if (A != 0) lg = __builtin_ctzll(uint64_t(A));
+                while (A + (int64_t(1) << lg) > B) --lg;
+                const int n = r - lg;
+                const int64_t k = A >> lg;
+                const double ps = std::ldexp(double(k), -n), pe = std::ldexp(double(k + 1), -n);
+                if (n > J) {
+                    if (!stored) {
+                        if (!left) l0 = ps;
+                        left = true;
+                        l1 = pe;
+                    } else {
+                        if (!right) r0 = ps;
+                        right = true;
+                        r1 = pe;
+                    }
+                } else if (!right && cnt < 2 * (J + 1)) {
+                    stored = true;
+                    P[kPlanHead + cnt] = node_offset(n, n_streams, s, k);
+                    ++cnt;
+                }
+                A += int64_t(1) << lg;
+            }
+            if (left) {
+                runs[0] = lower_bound_dev(params, lo, hi, l0);
+                runs[1] = lower_bound_dev(params, lo, hi, l1);
+            }
+            if (right) {
+                runs[2] = lower_bound_dev(params, lo, hi, r0);
+                runs[3] = lower_bound_dev(params, lo, hi, r1);
+            }
+        }
+    }
+    P[0] = valid ? s : -1;
+    P[1] = runs[0];
+    P[2] = runs[1];
+    P[3] = runs[2];
+    P[4] = runs[3];
+    P[5] = cnt;
+}
+
 template <typename T, int W, int D, int P>
 __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
         T* __restrict__ out, const T* __restrict__ cache, const T* __restrict__ incr,
-        const double* __restrict__ params, const int64_t* __restrict__ offsets, int64_t n_streams,
-        int r, int J, const int64_t* __restrict__ q_stream, const double* __restrict__ q_inf,
-        const double* __restrict__ q_sup, int64_t n_queries)
+        const int64_t* __restrict__ plans, int rec, int64_t n_queries)
 {
     using Sh = chen::Shape<W, D, P>;
     static_assert(Sh::NP <= 128, "a run fits two waves");
@@ -338,62 +362,20 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
     T* S = reinterpret_cast<T*>(smem);
     T* N = S + kT;
     T* R = N + kT;
-    __shared__ int64_t pk[kMaxPieces];
-    __shared__ int pn[kMaxPieces];
-    __shared__ int64_t runs[4];
-    __shared__ int coarse[2];
-    __shared__ int64_t qs;
-    __shared__ int64_t qlo, qhi;
-    __shared__ double bvals[4];
-    __shared__ int bneed[4];
+    __shared__ int64_t prec[kPlanHead + 2 * 25];
     __shared__ T XT[2 * kQueryChunk * kDW];
     const int half = threadIdx.x / 128;   // 0: left run, 1: right run
     const int q_lane = threadIdx.x % 128;
-    static_assert(kBlock / 64 >= 4, "one wave per run bound");
 
     for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
-        if (threadIdx.x == 0) {
-            const int64_t s = q_stream[q];
-            const bool valid = s >= 0 && s < n_streams;
-            int c[2] = {0, 0};
-            int need[4] = {0, 0, 0, 0};
-            double vals[4] = {0.0, 0.0, 0.0, 0.0};
-            int64_t lo = 0, hi = 0;
-            if (valid) {
-                lo = offsets[s];
-                hi = offsets[s + 1];
-                const double scale = std::ldexp(1.0, r);
-                double a = std::floor(q_inf[q] * scale), e = std::floor(q_sup[q] * scale);
-                a = std::min(std::max(a, 0.0), scale);
-                e = std::min(std::max(e, 0.0), scale);
-                if (a < e) plan_cover(int64_t(a), int64_t(e), r, J, pk, pn, vals, need, c);
-            }
-            for (int i = 0; i < 4; ++i) {
-                bvals[i] = vals[i];
-                bneed[i] = need[i];
-            }
-            qlo = lo;
-            qhi = hi;
-            coarse[0] = c[0];
-            coarse[1] = c[1];
-            qs = valid ? s : 0;
-        }
+        for (int i = threadIdx.x; i < rec; i += kBlock) prec[i] = plans[q * rec + i];
         __syncthreads();
-        {
-            // the run bounds: wave i searches bound i
-            const int wv = int(threadIdx.x / 64);
-            if (wv < 4) {
-                const int64_t b = bneed[wv] ? lower_bound_wave(params, qlo, qhi, bvals[wv]) : qlo;
-                if (threadIdx.x % 64 == 0) runs[wv] = b;
-            }
-        }
-        __syncthreads();
-        const int64_t s = qs;
-        const int c0 = coarse[0], c1 = coarse[1];
+        const int64_t runs[4] = {prec[1], prec[2], prec[3], prec[4]};
+        const int c0 = 0, c1 = int(prec[5]);
         // first stored node: its loads overlap the fine runs
         T pre[kPer];
         auto load_node = [&](int p) {
-            const T* node = cache + node_offset(pn[p], n_streams, s, pk[p]) * kT;
+            const T* node = cache + prec[kPlanHead + p] * kT;
             chen::static_for<kPer>([&](auto k0) {
                 constexpr int k = int(decltype(k0)::value);
                 const int i = threadIdx.x + k * kBlock;
@@ -495,9 +477,17 @@ bool launch_query_fast(void* out, const void* cache, const void* incr, const dou
         hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)) != hipSuccess)
         return false;
+    const int rec = plan_record_words(J);
+    int64_t* plans = nullptr;
+    if (workspace_alloc((void**)&plans, size_t(n_queries) * size_t(rec) * sizeof(int64_t), st) !=
+        hipSuccess)
+        return false;
+    const uint32_t pgrid = uint32_t((n_queries + 255) / 256);
+    hipLaunchKernelGGL(dyadic_query_plan_kernel, dim3(pgrid), dim3(256), 0, st, plans, rec,
+                       params, offsets, n_streams, r, J, q_stream, q_inf, q_sup, n_queries);
     hipLaunchKernelGGL((dyadic_query_fast_kernel<T, W, D, P>), dim3(grid), dim3(kBlock), dyn, st,
-                       (T*)out, (const T*)cache, (const T*)incr, params, offsets, n_streams, r,
-                       J, q_stream, q_inf, q_sup, n_queries);
+                       (T*)out, (const T*)cache, (const T*)incr, plans, rec, n_queries);
+    (void)hipFreeAsync(plans, st);
     return true;
 }
 
