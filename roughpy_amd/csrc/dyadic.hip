@@ -367,8 +367,16 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
     const int half = threadIdx.x / 128;   // 0: left run, 1: right run
     const int q_lane = threadIdx.x % 128;
 
+    // each thread below rec holds one word of the next query's plan
+    int64_t nxt = 0;
+    if (int(threadIdx.x) < rec && int64_t(blockIdx.x) < n_queries)
+        nxt = plans[int64_t(blockIdx.x) * rec + threadIdx.x];
     for (int64_t q = blockIdx.x; q < n_queries; q += gridDim.x) {
-        for (int i = threadIdx.x; i < rec; i += kBlock) prec[i] = plans[q * rec + i];
+        if (int(threadIdx.x) < rec) {
+            prec[threadIdx.x] = nxt;
+            const int64_t qn = q + gridDim.x;
+            if (qn < n_queries) nxt = plans[qn * rec + threadIdx.x];
+        }
         __syncthreads();
         const int64_t runs[4] = {prec[1], prec[2], prec[3], prec[4]};
         const int c0 = 0, c1 = int(prec[5]);
