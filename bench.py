@@ -537,6 +537,40 @@ def graph_leg(torch, reps=200):
             "eager_us_per_call": eager_ms * 1e3, "graph_us_per_replay": graph_ms * 1e3}
 
 
+def ref_cpu_rates(w, d, ops, target_seconds=0.4):
+    """The reference's own roughpy_compute kernels (oracle/_ref/libref.so,
+    compiled from the reference headers) on one host core, float64, over a
+    bounded sample of tensors of the leg's shape: elements per second for
+    each named operation.  ops: {label: (harness function, operand count)}
+    where the harness function takes (out, operands..., width, depth, n)."""
+    from oracle import oracle as O
+    R = O.ref_lib()
+    if R is None:
+        return None
+    size = O.tensor_size(w, d)
+    rng = np.random.default_rng(SEED + 90)
+    out = {}
+    for label, (name, n_in) in ops.items():
+        f = getattr(R, f"{name}_f64")
+
+        def run(n):
+            ins = [np.ascontiguousarray(rng.uniform(-1, 1, size=(n, size)) / 4)
+                   for _ in range(n_in)]
+            for a in ins:
+                a[:, 0] = 1.0
+            o = np.zeros((n, size))
+            t0 = time.perf_counter()
+            f(o.ctypes.data, *[a.ctypes.data for a in ins], w, d, n)
+            return time.perf_counter() - t0
+
+        dt = run(2)
+        n = int(max(2, min(4096, target_seconds / max(dt / 2, 1e-9))))
+        dt = run(n)
+        out[label] = {"value": n / dt, "unit": "elements/s", "cores": 1, "kind": "reference",
+                      "sample": f"{n} float64 tensors, {dt:.2f} s"}
+    return out
+
+
 def cpu_baseline(inc, offs, target_seconds=10.0):
     """Reference CPU path on a bounded sample: the reference's own
     roughpy_compute ft_fmexp per increment (oracle/_ref/libref.so, built from
@@ -702,6 +736,18 @@ def main():
         line["graph_replay"] = graph_leg(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(inc, offs)
+        # the reference CPU kernels beside the secondary legs (one core, float64)
+        if "fma_w4_d6" in line:
+            line["fma_w4_d6"]["cpu_reference"] = ref_cpu_rates(
+                4, 6, {"ft_fma": ("ref_ft_fma", 2)})
+        if "dense_w5_d5" in line:
+            line["dense_w5_d5"]["cpu_reference"] = ref_cpu_rates(5, 5, {
+                "inplace_mul": ("ref_ft_inplace_mul", 1), "exp": ("ref_ft_exp", 1),
+                "fmexp": ("ref_ft_fmexp", 2), "log": ("ref_ft_log", 1),
+                "antipode": ("ref_antipode", 1), "adjoint_left_mul": ("ref_adj_lmul", 2)})
+        if "shuffle_w4_d6" in line:
+            line["shuffle_w4_d6"]["cpu_reference"] = ref_cpu_rates(4, 6, {
+                "st_fma": ("ref_st_fma", 2), "st_adj_mul": ("ref_st_adj_mul", 2)})
     print(json.dumps(line), flush=True)
 
 

@@ -88,6 +88,9 @@ def ref_lib() -> Optional[ctypes.CDLL]:
         getattr(L, f"ref_ft_log_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
         getattr(L, f"ref_antipode_{sfx}").argtypes = [_ptr, _ptr, _c_int, _c_int, _c_i64]
         getattr(L, f"ref_adj_lmul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_st_fma_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int, _c_i64]
+        getattr(L, f"ref_st_adj_mul_{sfx}").argtypes = [_ptr, _ptr, _ptr, _c_int, _c_int,
+                                                        _c_i64]
         getattr(L, f"ref_signature_batch_{sfx}").argtypes = [
             _ptr, _ptr, _ptr, _c_i64, _c_int, _c_int, _c_int]
     return L
