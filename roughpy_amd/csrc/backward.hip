@@ -380,7 +380,10 @@ void launch_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n
                        (T*)grad, (const T*)incr, offsets, (const T*)sig, (const T*)gsig, b);
 }
 
-// Shapes with a 64-lane prefix split whose lane state fits in registers.
+// Shapes with a 64-lane prefix split (one wave per stream).  For float64 at
+// (8,4), (4,6) and (2,10) the lane state takes all 256 VGPRs (at (4,6) the
+// compiler also keeps part of it in AGPRs), which still beats splitting the
+// stream over several waves that must meet at a barrier every step.
 template <typename T>
 bool try_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n, const void* sig,
               const void* gsig, const DevBasis& b, hipStream_t st)
@@ -391,15 +394,9 @@ bool try_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n, c
         launch_wave<T, W, D, P>(grad, incr, offsets, n, sig, gsig, b, st); \
         return true;                                                       \
     }
-    RPY_WAVE(8, 2, 2) RPY_WAVE(8, 3, 2)
-    RPY_WAVE(4, 3, 3) RPY_WAVE(4, 4, 3) RPY_WAVE(4, 5, 3)
-    RPY_WAVE(2, 6, 6) RPY_WAVE(2, 7, 6) RPY_WAVE(2, 8, 6) RPY_WAVE(2, 9, 6)
-    if constexpr (std::is_same<T, float>::value) {
-        RPY_WAVE(8, 4, 2) RPY_WAVE(4, 6, 3) RPY_WAVE(2, 10, 6)
-    } else {
-        // wider splits where the double state would not fit one wave's lanes
-        RPY_WAVE(8, 4, 3) RPY_WAVE(4, 6, 4) RPY_WAVE(2, 10, 7)
-    }
+    RPY_WAVE(8, 2, 2) RPY_WAVE(8, 3, 2) RPY_WAVE(8, 4, 2)
+    RPY_WAVE(4, 3, 3) RPY_WAVE(4, 4, 3) RPY_WAVE(4, 5, 3) RPY_WAVE(4, 6, 3)
+    RPY_WAVE(2, 6, 6) RPY_WAVE(2, 7, 6) RPY_WAVE(2, 8, 6) RPY_WAVE(2, 9, 6) RPY_WAVE(2, 10, 6)
 #undef RPY_WAVE
     return false;
 }

@@ -192,6 +192,8 @@ def test_kernel_prefix_choice():
     L = _capi.lib()
     assert L.rpy_debug_signature_prefix(5, 5, 0) == 2
     assert L.rpy_debug_signature_prefix(5, 5, 1) == 3
+    assert L.rpy_debug_signature_prefix(8, 4, 1) == 2      # 75 float64 values per lane
+    assert L.rpy_debug_signature_prefix(4, 6, 1) == 4      # P = 3 would hold 88
     assert L.rpy_debug_signature_prefix(9, 3, 1) == 0      # generic kernel
 
 

@@ -195,8 +195,8 @@ def test_signature_vjp_kernels_vs_host_lane_algebra(gpu, rng, w, d):
     (ag.signature(x32, torch.from_numpy(offs).cuda(), w, d)
      * torch.from_numpy(G.astype(np.float32)).cuda()).sum().backward()
     got32 = x32.grad.cpu().numpy()
-    # the prefix length each dtype's register kernel uses for the shape
-    prefix64 = {(8, 2): 2, (8, 3): 2, (8, 4): 3, (4, 3): 3, (4, 5): 3, (4, 6): 4, (2, 6): 6,
+    # the prefix length the float64 register kernel uses for the shape
+    prefix64 = {(8, 2): 2, (8, 3): 2, (8, 4): 2, (4, 3): 3, (4, 5): 3, (4, 6): 3, (2, 6): 6,
                 (2, 7): 6}
     emulated = (w, d) in prefix64
     for s in range(n):
