@@ -755,9 +755,11 @@ class PiecewiseAbelianStream:
         return [(RealInterval(a, b), host[i]) for i, (a, b) in enumerate(zip(self._inf, self._sup))]
 
     def _terms(self, c: float, d: float):
-        """(piece, multiplier) pairs for the dyadic interval [c, d)."""
+        """(piece, multiplier) pairs for the dyadic interval [c, d).  Pieces are
+        consecutive, so the scan starts at the first piece ending after c."""
         out = []
-        for i in range(len(self._inf)):
+        start = int(np.searchsorted(self._sup, c, side="right")) if self._sup.size else 0
+        for i in range(start, len(self._inf)):
             pa, pb = self._inf[i], self._sup[i]
             if pa >= d:
                 break
