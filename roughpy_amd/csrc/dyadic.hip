@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_kernel(
     }
 }
 
-constexpr int kQueryChunk = 24;  // fine-run rows staged per pass and side
+constexpr int kQueryChunk = 48;  // fine-run rows staged per pass
 
 // offset of level k in a dense tensor of width W
 template <int W>
@@ -221,11 +221,6 @@ constexpr int dbeg(int k)
     return s;
 }
 
-// Cover of a query split into [fine run][stored nodes][fine run]: pieces
-// finer than the stored levels sit only at the two ends of the cover, and the
-// fine pieces of one end are adjacent, so each end is one contiguous run of
-// rows.  Fills the row ranges of the runs and the index range of the stored
-// pieces; returns the piece count.
 // Unrolled S <- S (x) N for a compile-time shape, both tensors in LDS: for
 // each output level (top-down, barrier per level) every thread takes outputs
 // idx = tid + 256 q and sums lhs degree m-i times rhs degree i, i = 0..m
@@ -263,14 +258,6 @@ RPY_FI void wg_mul_inplace_t(T* S, const T* N)
     __syncthreads();
 }
 
-// Queries for shapes whose prefix split has at most 128 lanes.  Per query:
-// 1. thread 0 splits the cover into [left fine run][stored nodes][right run];
-// 2. the rows of both runs are staged chunk by chunk in LDS as scaled tables
-//    (x_a / d) by the whole workgroup; threads 0..127 step the left run and
-//    128..255 the right run, each lane holding its prefix share of the run's
-//    signature in registers (chen_core.hpp), then write it to S and R;
-// 3. S is multiplied by each stored node (the next node's coefficients are
-//    loaded into registers while the current product runs) and by R.
 // Query plans, one thread per query, computed ahead of the query kernel so
 // that a query's workgroup starts from one load.  The cover of [A, B) is
 // [fine run][stored nodes][fine run] (piece sizes grow, then shrink), so one
@@ -348,24 +335,97 @@ __global__ __launch_bounds__(256) void dyadic_query_plan_kernel(
     P[5] = cnt;
 }
 
+// Queries for shapes whose prefix split has at most 128 lanes.  Per query:
+// 1. the query's plan record (below) gives [left fine run][stored nodes]
+//    [right run];
+// 2. the left run's rows are staged chunk by chunk in LDS as scaled tables
+//    (x_a / d) by the whole workgroup, and lanes 0..NP-1 step through them,
+//    each holding its prefix share of the run's signature in registers
+//    (chen_core.hpp), then write it to S;
+// 3. S is multiplied by each stored node (the next node's coefficients are
+//    loaded into registers while the current product runs);
+// 4. the lanes load S back into registers, step through the right run the
+//    same way (S (x) exp(x_1) (x) ... equals S times the run's signature) and
+//    write the result to the output.
+// Two tensors of LDS per workgroup, so four workgroups share a CU at width 5
+// depth 5 in float32.
 template <typename T, int W, int D, int P>
 __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
         T* __restrict__ out, const T* __restrict__ cache, const T* __restrict__ incr,
         const int64_t* __restrict__ plans, int rec, int64_t n_queries)
 {
     using Sh = chen::Shape<W, D, P>;
-    static_assert(Sh::NP <= 128, "a run fits two waves");
+    static_assert(Sh::NP <= kBlock, "a run's lanes fit the workgroup");
     constexpr int kT = dbeg<W>(D + 1);                  // tensor size
     constexpr int kPer = (kT + kBlock - 1) / kBlock;    // node values per thread
     constexpr int kDW = D * W;
     extern __shared__ __align__(16) unsigned char smem[];
     T* S = reinterpret_cast<T*>(smem);
     T* N = S + kT;
-    T* R = N + kT;
     __shared__ int64_t prec[kPlanHead + 2 * 25];
-    __shared__ T XT[2 * kQueryChunk * kDW];
-    const int half = threadIdx.x / 128;   // 0: left run, 1: right run
-    const int q_lane = threadIdx.x % 128;
+    __shared__ T XT[kQueryChunk * kDW];
+    const int lane = threadIdx.x;
+    const bool active = lane < Sh::NP;
+    int u[P];
+    {
+        int t = lane;
+        for (int j = P - 1; j >= 0; --j) {
+            u[j] = t % W;
+            t /= W;
+        }
+    }
+
+    // one run of rows [r0, r1) stepped into the lanes' registers
+    auto run = [&](chen::Lane<T, W, D, P>& st, int64_t r0, int64_t r1) {
+        for (int64_t cs = r0; cs < r1; cs += kQueryChunk) {
+            const int n = int(min<int64_t>(kQueryChunk, r1 - cs));
+            for (int e = threadIdx.x; e < n * kDW; e += kBlock) {
+                const int row = e / kDW;
+                const int da = e - row * kDW;
+                const int d = da / W + 1;
+                const int a = da - (d - 1) * W;
+                const T x = incr[(cs + row) * W + a];
+                XT[e] = (d == 1) ? x : x * (T(1) / T(d));
+            }
+            __syncthreads();
+            if (active)
+                for (int i = 0; i < n; ++i)
+                    st.step(chen::ScaledRow<T, W, D, P>{XT + i * kDW, u});
+            __syncthreads();
+        }
+    };
+    // lane state <-> dense tensor: the lane's subtree, and the path values
+    // from the lanes that start each prefix group
+    auto store = [&](const chen::Lane<T, W, D, P>& st, T* dst) {
+        chen::static_for<D - P + 1>([&](auto j0) {
+            constexpr int k = P + int(decltype(j0)::value);
+            constexpr int nk = int(ipow(W, k - P));
+            chen::static_for<nk>([&](auto v0) {
+                constexpr int v = int(decltype(v0)::value);
+                dst[dbeg<W>(k) + lane * nk + v] = st.sub[Sh::off(k) + v];
+            });
+        });
+        chen::static_for<P>([&](auto k0) {
+            constexpr int k = int(decltype(k0)::value);
+            constexpr int grp = int(ipow(W, P - k));
+            if (lane % grp == 0) dst[dbeg<W>(k) + lane / grp] = st.path[k];
+        });
+    };
+    auto fetch = [&](chen::Lane<T, W, D, P>& st, const T* src) {
+        chen::static_for<D - P + 1>([&](auto j0) {
+            constexpr int k = P + int(decltype(j0)::value);
+            constexpr int nk = int(ipow(W, k - P));
+            chen::static_for<nk>([&](auto v0) {
+                constexpr int v = int(decltype(v0)::value);
+                st.sub[Sh::off(k) + v] = src[dbeg<W>(k) + lane * nk + v];
+            });
+        });
+        chen::static_for<P>([&](auto k0) {
+            constexpr int k = int(decltype(k0)::value);
+            constexpr int grp = int(ipow(W, P - k));
+            st.path[k] = src[dbeg<W>(k) + lane / grp];
+        });
+    };
 
     // each thread below rec holds one word of the next query's plan
     int64_t nxt = 0;
@@ -378,9 +438,9 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
             if (qn < n_queries) nxt = plans[qn * rec + threadIdx.x];
         }
         __syncthreads();
-        const int64_t runs[4] = {prec[1], prec[2], prec[3], prec[4]};
-        const int c0 = 0, c1 = int(prec[5]);
-        // first stored node: its loads overlap the fine runs
+        const int64_t l0 = prec[1], l1 = prec[2], rr0 = prec[3], rr1 = prec[4];
+        const int nn = int(prec[5]);
+        // first stored node: its loads overlap the left run
         T pre[kPer];
         auto load_node = [&](int p) {
             const T* node = cache + prec[kPlanHead + p] * kT;
@@ -390,73 +450,31 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
                 if (i < kT) pre[k] = node[i];
             });
         };
-        if (c0 < c1) load_node(c0);
+        if (nn > 0) load_node(0);
 
-        const int64_t l0 = runs[0], l1 = runs[1], rr0 = runs[2], rr1 = runs[3];
-        const int64_t longest = max(l1 - l0, rr1 - rr0);
-        int u[P];
-        {
-            int t = q_lane;
-            for (int j = P - 1; j >= 0; --j) {
-                u[j] = t % W;
-                t /= W;
-            }
-        }
         chen::Lane<T, W, D, P> st;
         st.init_identity();
-        for (int64_t cs = 0; cs < longest; cs += kQueryChunk) {
-            const int nl = int(min<int64_t>(kQueryChunk, max<int64_t>(l1 - l0 - cs, 0)));
-            const int nr = int(min<int64_t>(kQueryChunk, max<int64_t>(rr1 - rr0 - cs, 0)));
-            for (int e = threadIdx.x; e < (nl + nr) * kDW; e += kBlock) {
-                const int side = e >= nl * kDW;
-                const int f = side ? e - nl * kDW : e;
-                const int row = f / kDW;
-                const int da = f - row * kDW;
-                const int d = da / W + 1;
-                const int a = da - (d - 1) * W;
-                const int64_t grow = (side ? rr0 : l0) + cs + row;
-                const T x = incr[grow * W + a];
-                XT[side * kQueryChunk * kDW + f] = (d == 1) ? x : x * (T(1) / T(d));
-            }
-            __syncthreads();
-            if (q_lane < Sh::NP) {
-                const int n = half == 0 ? nl : nr;
-                const T* tab = XT + half * kQueryChunk * kDW;
-                for (int i = 0; i < n; ++i)
-                    st.step(chen::ScaledRow<T, W, D, P>{tab + i * kDW, u});
-            }
-            __syncthreads();
-        }
-        if (q_lane < Sh::NP) {
-            T* dst = half == 0 ? S : R;
-            chen::static_for<D - P + 1>([&](auto j0) {
-                constexpr int k = P + int(decltype(j0)::value);
-                constexpr int nk = int(ipow(W, k - P));
-                chen::static_for<nk>([&](auto v0) {
-                    constexpr int v = int(decltype(v0)::value);
-                    dst[dbeg<W>(k) + q_lane * nk + v] = st.sub[Sh::off(k) + v];
-                });
-            });
-            chen::static_for<P>([&](auto k0) {
-                constexpr int k = int(decltype(k0)::value);
-                constexpr int grp = int(ipow(W, P - k));
-                if (q_lane % grp == 0) dst[dbeg<W>(k) + q_lane / grp] = st.path[k];
-            });
-        }
-        for (int p = c0; p < c1; ++p) {
+        run(st, l0, l1);
+        if (active) store(st, S);
+        for (int p = 0; p < nn; ++p) {
             chen::static_for<kPer>([&](auto k0) {
                 constexpr int k = int(decltype(k0)::value);
                 const int i = threadIdx.x + k * kBlock;
                 if (i < kT) N[i] = pre[k];
             });
-            __syncthreads();  // N staged; S and R written by the run lanes
-            if (p + 1 < c1) load_node(p + 1);
+            __syncthreads();  // N staged; S written by the lanes
+            if (p + 1 < nn) load_node(p + 1);
             wg_mul_inplace_t<T, W, D>(S, N);
         }
-        if (c0 >= c1) __syncthreads();  // S and R written by the run lanes
-        if (rr1 > rr0) wg_mul_inplace_t<T, W, D>(S, R);
+        if (nn == 0) __syncthreads();  // S written by the lanes
         T* o = out + q * kT;
-        for (int i = threadIdx.x; i < kT; i += kBlock) o[i] = S[i];
+        if (rr1 > rr0) {
+            if (active) fetch(st, S);
+            run(st, rr0, rr1);
+            if (active) store(st, o);
+        } else {
+            for (int i = threadIdx.x; i < kT; i += kBlock) o[i] = S[i];
+        }
         __syncthreads();
     }
 }
@@ -470,16 +488,16 @@ size_t device_lds_limit()
     return size_t(v);
 }
 
-// Launches the fast query kernel if its LDS (three tensors plus the staged
-// run tables) fits the device; returns false to fall back.
+// Launches the fast query kernel if its LDS (two tensors plus the staged
+// run table) fits the device; returns false to fall back.
 template <typename T, int W, int D, int P>
 bool launch_query_fast(void* out, const void* cache, const void* incr, const double* params,
                        const int64_t* offsets, int64_t n_streams, int r, int J,
                        const int64_t* q_stream, const double* q_inf, const double* q_sup,
                        int64_t n_queries, uint32_t grid, hipStream_t st)
 {
-    const size_t dyn = 3 * size_t(dbeg<W>(D + 1)) * sizeof(T);
-    const size_t stat = 2 * size_t(kQueryChunk) * D * W * sizeof(T) + 1024;
+    const size_t dyn = 2 * size_t(dbeg<W>(D + 1)) * sizeof(T);
+    const size_t stat = size_t(kQueryChunk) * D * W * sizeof(T) + 1024;
     if (dyn + stat > device_lds_limit()) return false;
     if (dyn + stat > 64 * 1024 &&
         hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
