@@ -1123,6 +1123,238 @@ int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_ten
     return RPY_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of left multiplication for a compile-time shape,
+// out(v) += sum_{u} op(u) arg(u v), both operands staged in LDS.
+// Output level od >= 1 is cut into tiles of the W words q c (c < W): the terms
+// of a tile are the op words u of degrees pd = D-od .. 0 (u ascending within
+// a degree, the reference's order, free_tensor_adjoint_left_mul.hpp:47-74),
+// each one op value times W contiguous arg values.  Level 0 is the dot
+// product of op and arg.  A tile (or level 0) with many terms is split over
+// an aligned group of G threads (a power of two <= 64) whose partial sums
+// meet in a butterfly; groups of all levels are packed into one virtual
+// thread range, heaviest levels first.
+// ---------------------------------------------------------------------------
+template <int W, int D>
+constexpr int adjf_terms(int od)
+{
+    return fdb<W>(D - od + 1);  // op words of degree 0 .. D - od
+}
+
+template <int W, int D>
+constexpr int adjf_group(int od)
+{
+    const int t = adjf_terms<W, D>(od);
+    int g = 1;
+    while (g < 64 && (t + 2 * g - 1) / (2 * g) >= 8) g *= 2;
+    return g;
+}
+
+template <int W>
+constexpr int adjf_tiles(int od)
+{
+    return od == 0 ? 1 : int(ipow(W, od - 1));
+}
+
+template <int W, int D>
+constexpr int adjf_vbase(int od)
+{
+    int v = 0;
+    for (int j = 0; j < od; ++j) v += adjf_group<W, D>(j) * adjf_tiles<W>(j);
+    return v;
+}
+
+template <typename T, int W, int D>
+__global__ __launch_bounds__(kBlock) void adj_lmul_fixed_kernel(T* __restrict__ out,
+                                                               const T* __restrict__ op,
+                                                               const T* __restrict__ arg,
+                                                               int64_t batch, int64_t os,
+                                                               int64_t ps, int64_t as)
+{
+    constexpr int kN = fdb<W>(D + 1);
+    constexpr int kNP = (kN + 3) / 4 * 4;
+    constexpr int kPer = (kN + kBlock - 1) / kBlock;   // staged values per thread
+    constexpr int kGrp = kPer < 16 ? kPer : 16;        // values per staging group
+    constexpr int kGroups = (kPer + kGrp - 1) / kGrp;
+    constexpr int kV = adjf_vbase<W, D>(D + 1);
+    constexpr int kPasses = (kV + kBlock - 1) / kBlock;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* P = reinterpret_cast<T*>(smem);
+    T* A = P + kNP;
+    for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
+        const T* p = op + e * ps;
+        const T* a = arg + e * as;
+        T* o = out + e * os;
+        // the global loads of a staging group (and of the output words this
+        // thread adds to) go out together, ahead of their LDS stores
+        T sp[kGrp], sa[kGrp];
+        auto load_grp = [&](auto g0) {
+            constexpr int g = int(decltype(g0)::value);
+            chen::static_for<kGrp>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + (g * kGrp + k) * kBlock;
+                if (g * kGrp + k < kPer && i < kN) {
+                    sp[k] = p[i];
+                    sa[k] = a[i];
+                }
+            });
+        };
+        auto store_grp = [&](auto g0) {
+            constexpr int g = int(decltype(g0)::value);
+            chen::static_for<kGrp>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + (g * kGrp + k) * kBlock;
+                if (g * kGrp + k < kPer && i < kN) {
+                    P[i] = sp[k];
+                    A[i] = sa[k];
+                }
+            });
+        };
+        load_grp(std::integral_constant<int, 0>{});
+        // pass ps_ covers virtual threads ps_ * kBlock + threadIdx.x; a level
+        // takes part in a pass only if their ranges meet (known at compile time)
+        T ov[kPasses][W];
+        int dst[kPasses];
+        chen::static_for<kPasses>([&](auto p0) {
+            constexpr int ps_ = int(decltype(p0)::value);
+            const int vt = ps_ * kBlock + int(threadIdx.x);
+            dst[ps_] = -1;
+            chen::static_for<D + 1>([&](auto od0) {
+                constexpr int od = int(decltype(od0)::value);
+                constexpr int G = adjf_group<W, D>(od);
+                constexpr int vb = adjf_vbase<W, D>(od);
+                constexpr int nv = G * adjf_tiles<W>(od);
+                if constexpr (vb < (ps_ + 1) * kBlock && vb + nv > ps_ * kBlock) {
+                    if (vt >= vb && vt < vb + nv) {
+                        const int r = vt - vb;
+                        const int q = r / G;
+                        if (r - q * G == 0) dst[ps_] = od == 0 ? 0 : fdb<W>(od) + q * W;
+                    }
+                }
+            });
+            const int lvl_w = dst[ps_] == 0 ? 1 : W;  // level 0 has one word
+            chen::static_for<W>([&](auto c) {
+                constexpr int ci = int(decltype(c)::value);
+                ov[ps_][ci] = (dst[ps_] >= 0 && ci < lvl_w) ? o[dst[ps_] + ci] : T(0);
+            });
+        });
+        __syncthreads();  // the previous tensor's readers are done
+        store_grp(std::integral_constant<int, 0>{});
+        chen::static_for<kGroups - 1>([&](auto g0) {
+            constexpr int g = int(decltype(g0)::value) + 1;
+            load_grp(std::integral_constant<int, g>{});
+            store_grp(std::integral_constant<int, g>{});
+        });
+        __syncthreads();
+        chen::static_for<kPasses>([&](auto p0) {
+            constexpr int ps_ = int(decltype(p0)::value);
+            const int vt = ps_ * kBlock + int(threadIdx.x);
+            T acc[W];
+            chen::static_for<W>([&](auto c) { acc[c] = T(0); });
+            int gw = 1;  // the lane's group size
+            chen::static_for<D + 1>([&](auto od0) {
+                constexpr int od = int(decltype(od0)::value);
+                constexpr int G = adjf_group<W, D>(od);
+                constexpr int vb = adjf_vbase<W, D>(od);
+                constexpr int nv = G * adjf_tiles<W>(od);
+                constexpr int T_ = adjf_terms<W, D>(od);
+                constexpr int chunk = (T_ + G - 1) / G;
+                constexpr int wod = int(ipow(W, od));
+                if constexpr (vb < (ps_ + 1) * kBlock && vb + nv > ps_ * kBlock) {
+                    if (vt >= vb && vt < vb + nv) {
+                        const int r = vt - vb;
+                        const int q = r / G;          // tile
+                        const int k = r - q * G;      // member of its group
+                        const int f0 = k * chunk;
+                        const int f1 = f0 + chunk < T_ ? f0 + chunk : T_;
+                        // flat term f -> degree pd (descending from D - od), word u
+                        chen::static_for<D - od + 1>([&](auto i0) {
+                            constexpr int pd = D - od - int(decltype(i0)::value);
+                            constexpr int F = fdb<W>(D - od + 1) - fdb<W>(pd + 1);
+                            constexpr int S = int(ipow(W, pd));
+                            const int lo = f0 - F > 0 ? f0 - F : 0;
+                            const int hi = f1 - F < S ? f1 - F : S;
+                            const T* Pp = P + fdb<W>(pd);
+                            if constexpr (od == 0) {
+                                const T* Ap = A + fdb<W>(pd);
+                                for (int u = lo; u < hi; ++u) acc[0] += Pp[u] * Ap[u];
+                            } else {
+                                const T* Ap = A + fdb<W>(od + pd) + q * W;
+                                for (int u = lo; u < hi; ++u) {
+                                    const T pv = Pp[u];
+                                    const T* ar = Ap + u * wod;
+                                    chen::static_for<W>([&](auto c) {
+                                        acc[c] += pv * ar[int(decltype(c)::value)];
+                                    });
+                                }
+                            }
+                        });
+                        gw = G;
+                    }
+                }
+            });
+            // group butterfly (groups are aligned powers of two inside a
+            // wave), skipped by waves with no group
+            if (__any(gw > 1)) {
+                for (int off = 1; off < 64; off *= 2) {
+                    chen::static_for<W>([&](auto c) {
+                        const T other = __shfl_xor(acc[c], off, 64);
+                        if (off < gw) acc[c] += other;
+                    });
+                }
+            }
+            if (dst[ps_] >= 0) {
+                const int lvl_w = dst[ps_] == 0 ? 1 : W;
+                chen::static_for<W>([&](auto c) {
+                    constexpr int ci = int(decltype(c)::value);
+                    if (ci < lvl_w) o[dst[ps_] + ci] = ov[ps_][ci] + acc[ci];
+                });
+            }
+        });
+    }
+}
+
+template <typename T, int W, int D>
+bool launch_adjl_fixed(void* out, const void* op, const void* arg, int64_t batch, int64_t os,
+                       int64_t ps, int64_t as, hipStream_t st)
+{
+    constexpr size_t lds = 2 * size_t((fdb<W>(D + 1) + 3) / 4 * 4) * sizeof(T);
+    if constexpr (lds > 128 * 1024) {
+        return false;
+    } else {
+        if constexpr (lds > 64 * 1024) {
+            // extended LDS (one workgroup per CU), reserved once per process
+            static const hipError_t err = hipFuncSetAttribute(
+                    (const void*)adj_lmul_fixed_kernel<T, W, D>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (err != hipSuccess) return false;
+        }
+        hipLaunchKernelGGL((adj_lmul_fixed_kernel<T, W, D>), dim3(grid_for(batch)), dim3(kBlock),
+                           lds, st, (T*)out, (const T*)op, (const T*)arg, batch, os, ps, as);
+        return true;
+    }
+}
+
+template <typename T>
+using AdjlFixed = bool (*)(void*, const void*, const void*, int64_t, int64_t, int64_t, int64_t,
+                           hipStream_t);
+
+template <typename T>
+AdjlFixed<T> find_adjl_fixed(int w, int d)
+{
+#define RPY_F(W, D) \
+    if (w == W && d == D) return &launch_adjl_fixed<T, W, D>;
+    RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
+    RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
+    RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
+    RPY_F(5, 2) RPY_F(5, 3) RPY_F(5, 4) RPY_F(5, 5)
+    RPY_F(6, 2) RPY_F(6, 3) RPY_F(6, 4) RPY_F(6, 5)
+    RPY_F(7, 2) RPY_F(7, 3) RPY_F(7, 4)
+    RPY_F(8, 2) RPY_F(8, 3) RPY_F(8, 4)
+#undef RPY_F
+    return nullptr;
+}
+
 int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
                               const rpy_tensor_basis* basis, int64_t batch, int64_t os,
                               int64_t ps, int64_t as, void* stream)
@@ -1133,6 +1365,17 @@ int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dt
     if ((rc = check_batch(batch, {os, ps, as}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !op || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    {
+        const auto f64 = find_adjl_fixed<double>(b.width, b.depth);
+        const auto f32 = find_adjl_fixed<float>(b.width, b.depth);
+        const bool done = dtype == RPY_F64
+                ? (f64 && f64(out, op, arg, batch, os, ps, as, as_stream(stream)))
+                : (f32 && f32(out, op, arg, batch, os, ps, as, as_stream(stream)));
+        if (done) {
+            RPY_HIP_CHECK(hipGetLastError());
+            return RPY_OK;
+        }
+    }
     dim3 grid(grid_for(batch)), block(kBlock);
     const size_t es = dtype == RPY_F64 ? 8 : 4;
     const size_t lds = 2 * size_t((b.size() + 3) / 4 * 4) * es;

@@ -499,10 +499,13 @@ bool launch_query_fast(void* out, const void* cache, const void* incr, const dou
     const size_t dyn = 2 * size_t(dbeg<W>(D + 1)) * sizeof(T);
     const size_t stat = size_t(kQueryChunk) * D * W * sizeof(T) + 1024;
     if (dyn + stat > device_lds_limit()) return false;
-    if (dyn + stat > 64 * 1024 &&
-        hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)) != hipSuccess)
-        return false;
+    if (dyn + stat > 64 * 1024) {
+        // extended LDS, reserved once per process and shape
+        static const hipError_t err =
+                hipFuncSetAttribute((const void*)dyadic_query_fast_kernel<T, W, D, P>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn));
+        if (err != hipSuccess) return false;
+    }
     const int rec = plan_record_words(J);
     int64_t* plans = nullptr;
     if (workspace_alloc((void**)&plans, size_t(n_queries) * size_t(rec) * sizeof(int64_t), st) !=
