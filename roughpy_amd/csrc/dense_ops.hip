@@ -390,15 +390,51 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
     for (int64_t e0 = int64_t(blockIdx.x) * E; e0 < batch; e0 += int64_t(gridDim.x) * E) {
         const int ne = int(batch - e0 < E ? batch - e0 : E);
         __syncthreads();  // the previous tensors' readers are done
-        for (int q = 0; q < ne; ++q) {
-            const T* a = arg + (e0 + q) * as;
-            for (int64_t c = threadIdx.x; c < cols; c += Block) s[q * cols + c] = a[c];
+        // eight loads per thread in flight before their LDS stores; a
+        // contiguous batch is staged as one flat range
+        if (as == cols) {
+            const T* a = arg + e0 * as;
+            const int64_t tot = int64_t(ne) * cols;
+            for (int64_t j0 = 0; j0 < tot; j0 += 8 * Block) {
+                T v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t j = j0 + threadIdx.x + int64_t(k) * Block;
+                    if (j < tot) v[k] = a[j];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t j = j0 + threadIdx.x + int64_t(k) * Block;
+                    if (j < tot) s[j] = v[k];
+                }
+            }
+        } else {
+            for (int q = 0; q < ne; ++q) {
+                const T* a = arg + (e0 + q) * as;
+                for (int64_t c0 = 0; c0 < cols; c0 += 8 * Block) {
+                    T v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int64_t c = c0 + threadIdx.x + int64_t(k) * Block;
+                        if (c < cols) v[k] = a[c];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int64_t c = c0 + threadIdx.x + int64_t(k) * Block;
+                        if (c < cols) s[q * cols + c] = v[k];
+                    }
+                }
+            }
         }
         __syncthreads();
         for (int64_t r = threadIdx.x; r < rows; r += Block) {
-            T acc[E];
+            // the output values this row adds to load while its entries run
+            T ov[E], acc[E];
 #pragma unroll
-            for (int q = 0; q < E; ++q) acc[q] = T(0);
+            for (int q = 0; q < E; ++q) {
+                ov[q] = q < ne ? out[(e0 + q) * os + r] : T(0);
+                acc[q] = T(0);
+            }
             const int64_t pe = offs[r + 1];
             int64_t p = offs[r];
             // entries in groups of 4: the group's index and value loads are
@@ -424,7 +460,7 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
             }
 #pragma unroll
             for (int q = 0; q < E; ++q)
-                if (q < ne) out[(e0 + q) * os + r] += acc[q];
+                if (q < ne) out[(e0 + q) * os + r] = ov[q] + acc[q];
         }
     }
 }
@@ -449,7 +485,20 @@ __global__ __launch_bounds__(kBlock) void antipode_lds_kernel(T* __restrict__ ou
     for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
         const T* a = arg + e * as;
         __syncthreads();  // the previous tensor's readers are done
-        for (uint32_t j = threadIdx.x; j < n; j += kBlock) sx[swz(j)] = a[j];
+        // eight loads per thread in flight before their LDS stores
+        for (uint32_t j0 = 0; j0 < n; j0 += 8 * kBlock) {
+            T v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t j = j0 + threadIdx.x + uint32_t(k) * kBlock;
+                if (j < n) v[k] = a[j];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t j = j0 + threadIdx.x + uint32_t(k) * kBlock;
+                if (j < n) sx[swz(j)] = v[k];
+            }
+        }
         __syncthreads();
         T* o = out + e * os;
         for (int d = 0; d <= b.depth; ++d) {
