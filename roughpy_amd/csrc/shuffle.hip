@@ -481,7 +481,11 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 constexpr int i = int(decltype(ic)::value);
                 wp[i] = 0;
                 if (i < k) {
-                    wp[i] = wpow_rt<W, D>(__builtin_ctz(t));
+                    if constexpr ((W & (W - 1)) == 0)
+                        wp[i] = 1u << (uint32_t(__builtin_ctz(t)) *
+                                       uint32_t(__builtin_ctz(uint32_t(W))));
+                    else
+                        wp[i] = wpow_rt<W, D>(__builtin_ctz(t));
                     t &= t - 1u;
                 }
             });
@@ -501,26 +505,49 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 }
             });
         }
-        for (uint32_t vh = vb; vh < ve; ++vh) {
-            const uint32_t ob = dbk + vh * W;
-            const uint32_t ab = dbn + U + V;
-            chen::static_for<W>([&](auto a) {
-                acc = fmaT(P[ob + a], A[ab + uint32_t(decltype(a)::value) * wp[0]], acc);
+        if constexpr ((W & (W - 1)) == 0) {
+            // W = 2^b: a level index holds letter j in bits [j b, j b + b), so
+            // the v-high offset V is vh's digits deposited into the bit fields
+            // of S's positions above its lowest, and vh + 1 is the masked
+            // increment ((V | ~M) + 1) & M over those fields
+            constexpr uint32_t b = uint32_t(__builtin_ctz(uint32_t(W)));
+            uint32_t M = 0;
+            chen::static_for<D>([&](auto jc) {
+                constexpr int j = int(decltype(jc)::value);
+                if (j < n && ((S >> j) & 1u)) M |= uint32_t(W - 1) << (uint32_t(j) * b);
             });
-            // odometer over the v-high digits
-            bool carry = true;
-            chen::static_for<D>([&](auto ic) {
-                constexpr int i = int(decltype(ic)::value);
-                if (i >= 1 && i < k && carry) {
-                    V += wp[i];
-                    if (++dv[i] == uint32_t(W)) {
-                        dv[i] = 0;
-                        V -= uint32_t(W) * wp[i];
-                    } else {
-                        carry = false;
+            M &= ~(uint32_t(W - 1) << (uint32_t(__builtin_ctz(S)) * b));
+            uint32_t ob = dbk + vb * W;
+            for (uint32_t vh = vb; vh < ve; ++vh) {
+                const uint32_t ab = dbn + U + V;
+                chen::static_for<W>([&](auto a) {
+                    acc = fmaT(P[ob + a], A[ab + uint32_t(decltype(a)::value) * wp[0]], acc);
+                });
+                V = ((V | ~M) + 1u) & M;
+                ob += uint32_t(W);
+            }
+        } else {
+            for (uint32_t vh = vb; vh < ve; ++vh) {
+                const uint32_t ob = dbk + vh * W;
+                const uint32_t ab = dbn + U + V;
+                chen::static_for<W>([&](auto a) {
+                    acc = fmaT(P[ob + a], A[ab + uint32_t(decltype(a)::value) * wp[0]], acc);
+                });
+                // odometer over the v-high digits
+                bool carry = true;
+                chen::static_for<D>([&](auto ic) {
+                    constexpr int i = int(decltype(ic)::value);
+                    if (i >= 1 && i < k && carry) {
+                        V += wp[i];
+                        if (++dv[i] == uint32_t(W)) {
+                            dv[i] = 0;
+                            V -= uint32_t(W) * wp[i];
+                        } else {
+                            carry = false;
+                        }
                     }
-                }
-            });
+                });
+            }
         }
         // next k-subset (Gosper)
         const uint32_t lowest = S & (0u - S);
