@@ -429,6 +429,7 @@ struct AdjPlan {
     int n_heavy_u;                                       // heavy outputs
     int parts[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1];  // parts[m][k]
     int chunk[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1];  // v-high values per part
+    int scount[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1]; // masks per part
     int per_u[kMaxShuffleDepth + 1];                     // sub-jobs per heavy output of level m
     int base[kMaxShuffleDepth + 2];                      // first sub-job of level m
     int ubase[kMaxShuffleDepth + 2];                     // first heavy output of level m
@@ -451,10 +452,34 @@ RPY_FI uint32_t db_rt(int j)
     return (wpow_rt<W, D + 1>(j) - 1u) / uint32_t(W - 1);
 }
 
-// sum over the k-subsets S of the m+k positions and v-high values
-// vh in [vb, ve) of op(v) arg(interleaving), v = vh W + a
+// The j-th k-subset of {0, .., n-1} in increasing mask order (the order of
+// Gosper's hack): the combinatorial number system, j = sum_i C(c_i, i) over
+// the members c_k > .. > c_1.
+RPY_FI uint32_t unrank_subset(uint32_t j, int k, int n)
+{
+    uint32_t S = 0;
+    for (int i = k; i >= 1; --i) {
+        int c = i - 1;       // C(i - 1, i) = 0 <= j
+        uint32_t bc = 0;     // C(c, i)
+        uint32_t bn = 1;     // C(c + 1, i)
+        while (c + 1 < n && bn <= j) {
+            ++c;
+            bc = bn;
+            bn = bn * uint32_t(c + 1) / uint32_t(c + 1 - i);
+        }
+        S |= 1u << c;
+        j -= bc;
+        n = c;
+    }
+    return S;
+}
+
+// sum over the k-subsets S of the m+k positions with ranks [sb, sb + sc)
+// (in increasing mask order) and v-high values vh in [vb, ve) of
+// op(v) arg(interleaving), v = vh W + a
 template <typename T, int W, int D>
-RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, uint32_t ve)
+RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, uint32_t ve,
+                 uint32_t sb = 0, uint32_t sc = ~0u)
 {
     const int n = m + k;
     if (k == 0) return P[0] * A[db_rt<W, D>(m) + u];
@@ -462,8 +487,8 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
     const uint32_t dbk = db_rt<W, D>(k);
     const uint32_t dbn = db_rt<W, D>(n);
     const uint32_t limit = 1u << n;
-    uint32_t S = (1u << k) - 1u;
-    while (S < limit) {
+    uint32_t S = sb == 0 ? (1u << k) - 1u : unrank_subset(sb, k, n);
+    for (uint32_t cnt = 0; cnt < sc && S < limit; ++cnt) {
         // u letters on the complement of S, lowest position = last letter
         uint32_t U = 0, ur = u;
         chen::static_for<D>([&](auto jc) {
@@ -604,11 +629,15 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
                 if (k == 0) {
                     s = adj_job<T, W, D>(P, A, m, u, 0, 0, 1);
                 } else {
+                    // part r: mask block r / vparts, v-high block r % vparts
                     const uint32_t vh_n = wpow_rt<W, D>(k - 1);
                     const uint32_t c = uint32_t(plan.chunk[m][k]);
-                    const uint32_t vb = r * c;
+                    const uint32_t vparts = (vh_n + c - 1) / c;
+                    const uint32_t ps = r / vparts;
+                    const uint32_t vb = (r - ps * vparts) * c;
                     const uint32_t ve = vb + c < vh_n ? vb + c : vh_n;
-                    s = vb < ve ? adj_job<T, W, D>(P, A, m, u, k, vb, ve) : T(0);
+                    const uint32_t sc = uint32_t(plan.scount[m][k]);
+                    s = vb < ve ? adj_job<T, W, D>(P, A, m, u, k, vb, ve, ps * sc, sc) : T(0);
                 }
                 part[t] = s;
             } else {
@@ -668,14 +697,23 @@ inline AdjPlan make_adj_plan(int W, int D, int block)
             for (int k = 0; m + k <= D; ++k) {
                 int parts = 1;
                 int chunk = 1;
+                int scount = 1;
                 if (k >= 1) {
+                    // split over blocks of masks first (each mask's setup is
+                    // then done once), then over blocks of v-high values
                     const double vh = wp(k - 1);
-                    parts = int(std::min(vh, std::max(1.0, std::ceil(binom(m + k, k) * wp(k) / grain))));
-                    chunk = int(std::ceil(vh / parts));
-                    parts = int(std::ceil(vh / chunk));
+                    const double masks = binom(m + k, k);
+                    const double target = std::min(
+                            masks * vh, std::max(1.0, std::ceil(masks * wp(k) / grain)));
+                    scount = int(std::ceil(masks / std::min(masks, target)));
+                    const int sparts = int(std::ceil(masks / scount));
+                    const double vp = std::min(vh, std::max(1.0, std::ceil(target / sparts)));
+                    chunk = int(std::ceil(vh / vp));
+                    parts = sparts * int(std::ceil(vh / chunk));
                 }
                 p.parts[m][k] = parts;
                 p.chunk[m][k] = chunk;
+                p.scount[m][k] = scount;
                 per += parts;
             }
             p.per_u[m] = per;
