@@ -84,7 +84,10 @@ int dyadic_dissection(double inf, double sup, int32_t tol, std::vector<DI>& out)
         return fail(RPY_ERR_INVALID_ARGUMENT, "interval too large for the resolution");
     DI begin = from_param(inf, tol);
     DI end = from_param(sup, tol);
-    std::vector<DI> left, right;
+    // per-thread scratch, reused across calls
+    static thread_local std::vector<DI> left, right;
+    left.clear();
+    right.clear();
     int guard = 0;
     while (!contains(begin, end)) {
         DI next = expand(begin);
@@ -103,7 +106,7 @@ int dyadic_dissection(double inf, double sup, int32_t tol, std::vector<DI>& out)
         end = next;
         if (++guard > 8192) return fail(RPY_ERR_INVALID_ARGUMENT, "dyadic dissection diverged");
     }
-    out = left;
+    out.assign(left.begin(), left.end());
     out.insert(out.end(), right.rbegin(), right.rend());
     return RPY_OK;
 }
@@ -115,22 +118,46 @@ int64_t lower_bound_param(const double* p, int64_t n, double v)
     return int64_t(std::lower_bound(p, p + n, v) - p);
 }
 
+// First index i >= b with p[i] >= v (p sorted, p[b - 1] < v): steps of 1, 2,
+// 4, .. from b, then a binary search inside the last step, so a short run
+// costs a few cache lines instead of a search over the whole array.
+int64_t gallop_param(const double* p, int64_t n, int64_t b, double v)
+{
+    int64_t lo = b, hi = b, step = 1;
+    while (hi < n && p[hi] < v) {
+        lo = hi + 1;
+        hi = b + step;
+        step *= 2;
+    }
+    if (hi > n) hi = n;
+    return int64_t(std::lower_bound(p + lo, p + hi, v) - p);
+}
+
 }  // namespace
 
 // Row ranges [b, e) of the sorted params that the query [inf, sup) at
-// `query_res` covers for a stream of resolution `stream_res`.
+// `query_res` covers for a stream of resolution `stream_res`.  A caller that
+// walks consecutive queries passes the end point and end row of the previous
+// one in (hint_hi, hint_e) (hint_e < 0: none) and gets this query's back.
 int select_rows(const double* params, int64_t n, int32_t stream_res, double inf, double sup,
-                int32_t query_res, std::vector<std::pair<int64_t, int64_t>>& ranges)
+                int32_t query_res, std::vector<std::pair<int64_t, int64_t>>& ranges,
+                double* hint_hi, int64_t* hint_e)
 {
     ranges.clear();
-    std::vector<DI> pieces;
+    static thread_local std::vector<DI> pieces;
     int rc = dyadic_dissection(inf, sup, query_res, pieces);
     if (rc) return rc;
+    // the pieces tile [inf, sup) left to right, so a piece starts where the
+    // previous one ended and its first row is the previous end row
+    double prev_hi = hint_hi ? *hint_hi : 0.0;
+    int64_t prev_e = hint_e ? *hint_e : -1;
     for (const DI& d : pieces) {
         const double lo = std::ldexp(double(d.k), -d.n);
         const double hi = std::ldexp(double(d.k + 1), -d.n);
-        const int64_t b = lower_bound_param(params, n, lo);
-        const int64_t e = lower_bound_param(params, n, hi);
+        const int64_t b = (prev_e >= 0 && lo == prev_hi) ? prev_e : lower_bound_param(params, n, lo);
+        const int64_t e = gallop_param(params, n, b, hi);
+        prev_hi = hi;
+        prev_e = e;
         if (b == e) continue;  // empty piece contributes nothing
         if (d.n <= stream_res) {
             ranges.push_back({b, e});
@@ -144,6 +171,10 @@ int select_rows(const double* params, int64_t n, int32_t stream_res, double inf,
                                   lower_bound_param(params, n, lhi)});
             }
         }
+    }
+    if (hint_hi && hint_e) {
+        *hint_hi = prev_hi;
+        *hint_e = prev_e;
     }
     return RPY_OK;
 }
@@ -183,7 +214,8 @@ int rpy_stream_select_rows(const double* params, int64_t n_params, int32_t strea
         if (!(params[i - 1] < params[i]))
             return fail(RPY_ERR_INVALID_ARGUMENT, "params must be strictly increasing");
     std::vector<std::pair<int64_t, int64_t>> r;
-    int rc = select_rows(params, n_params, stream_resolution, inf, sup, query_resolution, r);
+    int rc = select_rows(params, n_params, stream_resolution, inf, sup, query_resolution, r,
+                         nullptr, nullptr);
     if (rc) return rc;
     *n_ranges = int64_t(r.size());
     if (int64_t(r.size()) > max_ranges)
@@ -212,13 +244,15 @@ int rpy_stream_select_rows_batch(const double* params, int64_t n_params,
     std::vector<std::pair<int64_t, int64_t>> r;
     int64_t total = 0;
     offsets[0] = 0;
+    double hint_hi = 0.0;  // consecutive queries (a partition) share end points
+    int64_t hint_e = -1;
     for (int64_t q = 0; q < n_queries; ++q) {
         const double qi = infs[q], qs = sups[q];
         if (!(qs < support_inf || qi > support_sup)) {
             const double a = std::max(qi, support_inf), b = std::min(qs, support_sup);
             if (b - a != 0.0) {
                 int rc = select_rows(params, n_params, stream_resolution, a, b,
-                                     query_resolution, r);
+                                     query_resolution, r, &hint_hi, &hint_e);
                 if (rc) return rc;
                 for (const auto& pr : r) {
                     for (int64_t i = pr.first; i < pr.second; ++i) {

@@ -453,6 +453,7 @@ class LieIncrementStream:
         self.support = support         # (inf, sup)
         self.dtype = np.dtype(dtype)
         self._dev_rows = None
+        self._sel_buf = None  # row-selection indices, reused by simplify
 
     @classmethod
     def from_increments(cls, data, indices=None, *, width=None, depth=None, resolution=None,
@@ -671,7 +672,11 @@ class LieIncrementStream:
         mids = partition.intermediates()
         infs = np.array([partition.inf] + mids, dtype=np.float64)
         sups = np.array(mids + [partition.sup], dtype=np.float64)
-        idx, offs = self._select_batch(infs, sups, res)
+        # the index buffer is kept between calls: a fresh one costs a page
+        # fault per 4 KiB on first write, more than the selection itself
+        if self._sel_buf is None:
+            self._sel_buf = np.empty(self._params.shape[0] + 16, dtype=np.int64)
+        idx, offs = self._select_batch(infs, sups, res, self._sel_buf)
         t = _device.require_gpu()
         if self._dev_rows is None:
             self._dev_rows = _device.to_device(np.ascontiguousarray(self._rows), self.dtype)
@@ -681,22 +686,34 @@ class LieIncrementStream:
         del n
         return PiecewiseAbelianStream(infs, sups, lies, self.width, depth, res, self.support)
 
-    def _select_batch(self, infs, sups, resolution):
+    def _select_batch(self, infs, sups, resolution, buf=None):
         """Row indices (CSR) that each query [infs[q], sups[q]) covers at
-        `resolution`, one native call."""
+        `resolution`, one native call.  `buf` (int64, at least the stream's
+        length + 16) is filled instead of a new array, and the returned
+        indices are a view of it."""
         L = _capi.lib()
         params = np.ascontiguousarray(self._params, dtype=np.float64)
         nq = int(infs.shape[0])
         offs = np.zeros(nq + 1, dtype=np.int64)
         lo, hi = self.support
-        _capi.check(L.rpy_stream_select_rows_batch(
-            _capi.dblp(params), params.shape[0], self.resolution, lo, hi, _capi.dblp(infs),
-            _capi.dblp(sups), nq, int(resolution), _capi.i64p(offs), None, 0))
-        idx = np.zeros(max(1, int(offs[-1])), dtype=np.int64)
-        _capi.check(L.rpy_stream_select_rows_batch(
-            _capi.dblp(params), params.shape[0], self.resolution, lo, hi, _capi.dblp(infs),
-            _capi.dblp(sups), nq, int(resolution), _capi.i64p(offs), _capi.i64p(idx),
-            idx.shape[0]))
+
+        def call(idx):
+            return L.rpy_stream_select_rows_batch(
+                _capi.dblp(params), params.shape[0], self.resolution, lo, hi, _capi.dblp(infs),
+                _capi.dblp(sups), nq, int(resolution), _capi.i64p(offs),
+                None if idx is None else _capi.i64p(idx), 0 if idx is None else idx.shape[0])
+
+        # disjoint queries (a partition) select each row at most once, so one
+        # pass into a buffer of the stream's length usually suffices; more rows
+        # than that take a counting pass first
+        need = params.shape[0] + 16
+        idx = buf if buf is not None and buf.shape[0] >= need else np.empty(need, dtype=np.int64)
+        rc = call(idx)
+        if rc == _capi.RPY_ERR_SHAPE:
+            _capi.check(call(None))
+            idx = np.empty(max(1, int(offs[-1])), dtype=np.int64)
+            rc = call(idx)
+        _capi.check(rc)
         return idx[:int(offs[-1])], offs
 
     def log_signature(self, interval_or_inf=None, sup=None, resolution=None, *, depth=None,
