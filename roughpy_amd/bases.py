@@ -13,6 +13,10 @@ import numpy as np
 from . import _capi
 
 
+# (l2t or t2l, width, depth, Hall data) -> float64 SparseMatrix
+_MAP_CACHE = {}
+
+
 class TensorBasis:
     """Dense tensor basis: level k occupies [degree_begin[k], degree_begin[k+1])."""
 
@@ -226,7 +230,16 @@ class LieBasis:
         if dt not in cache:
             base = cache.get(np.dtype(np.float64))
             if base is None:
-                base = self._build(which)
+                # bases with the same Hall data share one matrix (and its
+                # device copy), so a fresh LieBasis does not rebuild it
+                key = (which, self._width, self._depth,
+                       np.ascontiguousarray(self._degree_begin, dtype=np.int64).tobytes(),
+                       np.ascontiguousarray(self._data[: self.size() + 1],
+                                            dtype=np.int64).tobytes())
+                base = _MAP_CACHE.get(key)
+                if base is None:
+                    base = self._build(which)
+                    _MAP_CACHE[key] = base
                 cache[np.dtype(np.float64)] = base
             if dt != np.dtype(np.float64):
                 cache[dt] = SparseMatrix(base.rows, base.cols, base.offsets, base.indices,

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "rpy_common.hpp"
@@ -179,6 +180,84 @@ int select_rows(const double* params, int64_t n, int32_t stream_res, double inf,
     return RPY_OK;
 }
 
+namespace {
+
+constexpr int64_t kMinQueriesPerThread = 2048;
+constexpr int kMaxSelectThreads = 16;
+
+struct BlockArgs {
+    const double* params;
+    int64_t n_params;
+    int32_t stream_res;
+    double support_inf, support_sup;
+    const double* infs;
+    const double* sups;
+    int32_t query_res;
+};
+
+// params[i - 1] < params[i] for every i, checked in blocks on host threads
+// when the array is long.
+bool params_increasing(const double* p, int64_t n)
+{
+    constexpr int64_t kMinPerThread = int64_t(1) << 18;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = int(std::max<int64_t>(
+            1, std::min<int64_t>({int64_t(hw ? hw : 1), int64_t(kMaxSelectThreads),
+                                  n / kMinPerThread})));
+    const int64_t chunk = (n + nt - 1) / nt;
+    auto check = [&](int t) {
+        const int64_t b = std::max<int64_t>(1, int64_t(t) * chunk);
+        const int64_t e = std::min(n, int64_t(t + 1) * chunk);
+        for (int64_t i = b; i < e; ++i)
+            if (!(p[i - 1] < p[i])) return false;
+        return true;
+    };
+    std::vector<char> ok(size_t(nt), 1);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back([&, t] { ok[size_t(t)] = check(t); });
+        } catch (...) {
+            ok[size_t(t)] = check(t);
+        }
+    }
+    ok[0] = check(0);
+    for (auto& th : pool) th.join();
+    for (char c : ok)
+        if (!c) return false;
+    return true;
+}
+
+// Queries [q0, q1) in order: with `ends`, the running count of selected rows
+// after each query (from 0 at q0); with `out`, the row indices in order.
+int select_block(const BlockArgs& a, int64_t q0, int64_t q1, int64_t* ends, int64_t* out)
+{
+    std::vector<std::pair<int64_t, int64_t>> r;
+    double hint_hi = 0.0;  // consecutive queries (a partition) share end points
+    int64_t hint_e = -1;
+    int64_t total = 0;
+    for (int64_t q = q0; q < q1; ++q) {
+        const double qi = a.infs[q], qs = a.sups[q];
+        if (!(qs < a.support_inf || qi > a.support_sup)) {
+            const double lo = std::max(qi, a.support_inf), hi = std::min(qs, a.support_sup);
+            if (hi - lo != 0.0) {
+                int rc = select_rows(a.params, a.n_params, a.stream_res, lo, hi, a.query_res, r,
+                                     &hint_hi, &hint_e);
+                if (rc) return rc;
+                for (const auto& pr : r) {
+                    if (out)
+                        for (int64_t i = pr.first; i < pr.second; ++i) out[total + i - pr.first] = i;
+                    total += pr.second - pr.first;
+                }
+            }
+        }
+        if (ends) ends[q - q0] = total;
+    }
+    return RPY_OK;
+}
+
+}  // namespace
+
 }  // namespace rpy_amd
 
 using namespace rpy_amd;
@@ -238,38 +317,65 @@ int rpy_stream_select_rows_batch(const double* params, int64_t n_params,
     if (n_queries < 0) return fail(RPY_ERR_SHAPE, "negative query count");
     if (!offsets || (n_params > 0 && !params) || (n_queries > 0 && (!infs || !sups)))
         return fail(RPY_ERR_INVALID_ARGUMENT, "null argument");
-    for (int64_t i = 1; i < n_params; ++i)
-        if (!(params[i - 1] < params[i]))
-            return fail(RPY_ERR_INVALID_ARGUMENT, "params must be strictly increasing");
-    std::vector<std::pair<int64_t, int64_t>> r;
-    int64_t total = 0;
+    if (!params_increasing(params, n_params))
+        return fail(RPY_ERR_INVALID_ARGUMENT, "params must be strictly increasing");
     offsets[0] = 0;
-    double hint_hi = 0.0;  // consecutive queries (a partition) share end points
-    int64_t hint_e = -1;
-    for (int64_t q = 0; q < n_queries; ++q) {
-        const double qi = infs[q], qs = sups[q];
-        if (!(qs < support_inf || qi > support_sup)) {
-            const double a = std::max(qi, support_inf), b = std::min(qs, support_sup);
-            if (b - a != 0.0) {
-                int rc = select_rows(params, n_params, stream_resolution, a, b,
-                                     query_resolution, r, &hint_hi, &hint_e);
-                if (rc) return rc;
-                for (const auto& pr : r) {
-                    for (int64_t i = pr.first; i < pr.second; ++i) {
-                        if (rows) {
-                            if (total >= max_rows)
-                                return fail(RPY_ERR_SHAPE, "more than %lld selected rows",
-                                            (long long)max_rows);
-                            rows[total] = i;
-                        }
-                        ++total;
-                    }
-                }
+    // Queries are independent: blocks of them run on host threads, first
+    // counting (each block's running counts land in its part of offsets),
+    // then, after a prefix over the blocks, writing their rows in place.
+    // The result is the same as one pass in query order.
+    const BlockArgs args{params, n_params, stream_resolution, support_inf, support_sup,
+                         infs, sups, query_resolution};
+    unsigned hw = std::thread::hardware_concurrency();
+    const int64_t by_work = (n_queries + kMinQueriesPerThread - 1) / kMinQueriesPerThread;
+    const int nt = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(hw ? hw : 1),
+                                                                int64_t(kMaxSelectThreads),
+                                                                by_work})));
+    const int64_t chunk = (n_queries + nt - 1) / std::max(nt, 1);
+    auto run = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        std::vector<int> rcs(size_t(nt), RPY_OK);
+        for (int t = 1; t < nt; ++t) {
+            try {
+                pool.emplace_back([&, t] { rcs[size_t(t)] = body(t); });
+            } catch (...) {
+                rcs[size_t(t)] = body(t);  // no thread to spare: run the block here
             }
         }
-        offsets[q + 1] = total;
+        rcs[0] = body(0);
+        for (auto& th : pool) th.join();
+        for (int t = 0; t < nt; ++t)
+            if (rcs[size_t(t)] != RPY_OK) return t;
+        return -1;
+    };
+    auto bounds = [&](int t) {
+        const int64_t q0 = std::min(n_queries, int64_t(t) * chunk);
+        return std::make_pair(q0, std::min(n_queries, q0 + chunk));
+    };
+    int bad = run([&](int t) {
+        const auto [q0, q1] = bounds(t);
+        return select_block(args, q0, q1, offsets + 1 + q0, nullptr);
+    });
+    if (bad >= 0) {
+        // the message of a worker's failure lives in that thread: the same
+        // block again on this thread fails the same way and sets it here
+        const auto [q0, q1] = bounds(bad);
+        return select_block(args, q0, q1, offsets + 1 + q0, nullptr);
     }
-    return RPY_OK;
+    int64_t base = 0;
+    for (int t = 0; t < nt; ++t) {
+        const auto [q0, q1] = bounds(t);
+        for (int64_t q = q0; q < q1; ++q) offsets[q + 1] += base;
+        if (q1 > q0) base = offsets[q1];
+    }
+    if (!rows) return RPY_OK;
+    if (base > max_rows)
+        return fail(RPY_ERR_SHAPE, "more than %lld selected rows", (long long)max_rows);
+    bad = run([&](int t) {
+        const auto [q0, q1] = bounds(t);
+        return select_block(args, q0, q1, nullptr, rows + offsets[q0]);
+    });
+    return bad >= 0 ? RPY_ERR_INVALID_ARGUMENT : RPY_OK;
 }
 
 }  // extern "C"

@@ -453,7 +453,8 @@ class LieIncrementStream:
         self.support = support         # (inf, sup)
         self.dtype = np.dtype(dtype)
         self._dev_rows = None
-        self._sel_buf = None  # row-selection indices, reused by simplify
+        self._sel_buf = None  # row-selection indices (pinned), reused by simplify
+        self._sel_pinned = None
 
     @classmethod
     def from_increments(cls, data, indices=None, *, width=None, depth=None, resolution=None,
@@ -672,15 +673,19 @@ class LieIncrementStream:
         mids = partition.intermediates()
         infs = np.array([partition.inf] + mids, dtype=np.float64)
         sups = np.array(mids + [partition.sup], dtype=np.float64)
-        # the index buffer is kept between calls: a fresh one costs a page
-        # fault per 4 KiB on first write, more than the selection itself
-        if self._sel_buf is None:
-            self._sel_buf = np.empty(self._params.shape[0] + 16, dtype=np.int64)
-        idx, offs = self._select_batch(infs, sups, res, self._sel_buf)
         t = _device.require_gpu()
+        # the index buffer is pinned host memory kept between calls: a fresh
+        # pageable one costs a page fault per 4 KiB on first write and a
+        # staged copy to the device, together more than the selection itself
+        if self._sel_buf is None:
+            self._sel_pinned = t.empty(self._params.shape[0] + 16, dtype=t.int64, pin_memory=True)
+            self._sel_buf = self._sel_pinned.numpy()
+        idx, offs = self._select_batch(infs, sups, res, self._sel_buf)
         if self._dev_rows is None:
             self._dev_rows = _device.to_device(np.ascontiguousarray(self._rows), self.dtype)
-        rows = self._dev_rows.index_select(0, t.from_numpy(idx).to(self._dev_rows.device))
+        # a blocking copy: the buffer is rewritten by the next call
+        dev_idx = self._sel_pinned[:idx.shape[0]].to(self._dev_rows.device)
+        rows = self._dev_rows.index_select(0, dev_idx)
         lb = LieBasis(self.width, depth)
         lies = log_signature_batch(rows, offs, self.width, depth, dtype=self.dtype, lie_basis=lb)
         del n

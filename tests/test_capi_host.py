@@ -301,3 +301,38 @@ def test_select_rows_batch_matches_single_queries():
             for q in range(a.shape[0]):
                 single = st._select((a[q], b[q]), qres)
                 assert np.array_equal(idx[offs[q]:offs[q + 1]], single), (res, qres, q)
+
+
+def test_select_rows_batch_threaded_blocks():
+    """Many queries run in blocks on host threads: the result equals one pass
+    in query order, for a partition (consecutive queries sharing end points),
+    for overlapping random queries (more rows than the stream holds, so the
+    one-pass buffer overflows and the counting pass sizes the output), and
+    for a failing block (its error message reaches the caller)."""
+    from roughpy_amd import streams as rs
+    rng = np.random.default_rng(11)
+    n = 20000
+    params = np.cumsum(rng.uniform(0.5, 1.5, size=n))
+    inc = rng.normal(size=(n, 2))
+    st = rs.LieIncrementStream.from_increments(inc, indices=params, depth=2, resolution=2)
+    lo, hi = st.support
+    cuts = np.sort(rng.uniform(lo, hi, size=9999))
+    a = np.concatenate([[lo], cuts])
+    b = np.concatenate([cuts, [hi]])
+    qa = rng.uniform(lo, hi, size=6000)
+    qb = qa + rng.uniform(0, 400, size=6000)
+    for xa, xb in ((a, b), (qa, qb)):
+        idx, offs = st._select_batch(xa, xb, 2)
+        assert offs.shape[0] == xa.shape[0] + 1
+        for q in range(0, xa.shape[0], 97):
+            single = st._select((xa[q], xb[q]), 2)
+            assert np.array_equal(idx[offs[q]:offs[q + 1]], single), q
+        counts = np.diff(offs)
+        ref = np.array([st._select((xa[q], xb[q]), 2).shape[0] for q in range(xa.shape[0])])
+        assert np.array_equal(counts, ref)
+    assert offs[-1] > n  # the overlapping queries select more rows than the stream has
+    # a NaN query end fails inside a worker block; the message comes back
+    bad_a = qa.copy()
+    bad_a[5000] = np.nan
+    with pytest.raises(ValueError, match="finite"):
+        st._select_batch(bad_a, qb, 2)
