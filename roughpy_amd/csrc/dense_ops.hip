@@ -1519,13 +1519,14 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
     if (batch == 0 || m->rows == 0) return RPY_OK;
     if (!out || !arg || !m->offsets || (m->nnz > 0 && (!m->indices || !m->data)))
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
-    // tensors staged per workgroup: 8 when they fit 48 KiB of LDS (several
-    // workgroups per CU), else up to 4 in at most 128 KiB with 512 threads
-    // (the matrix is re-read once per group, so reuse beats occupancy)
+    // tensors staged per workgroup: 8 when they fit 64 KiB of LDS (two or
+    // more workgroups per CU), else up to 4 in at most 128 KiB with 512
+    // threads (the matrix is re-read once per group, so reuse beats
+    // occupancy: 2 tensors per group measured slower than 4 for t2l at (5,5))
     const int64_t es = dtype == RPY_F64 ? 8 : 4;
     const int64_t per = m->cols * es;
     int E = 0;
-    if (per > 0 && 8 * per <= 48 * 1024)
+    if (per > 0 && 8 * per <= 64 * 1024)
         E = 8;
     else if (per > 0 && 4 * per <= 128 * 1024)
         E = 4;
