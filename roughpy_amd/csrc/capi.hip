@@ -278,13 +278,21 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
     double* lg = (double*)(ws + tel * elt + (dtype == RPY_F64 ? 0 : tel * 8));
     int rc = stream_signature_batch(sig, increments, offsets, n_streams, width, depth, dtype,
                                     stream);
-    if (!rc && dtype != RPY_F64) {
-        hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0, st, sig64,
-                           (const float*)sig, int64_t(tel));
-        if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
-    }
     rpy_tensor_basis tb{width, depth, nullptr};
-    if (!rc) rc = dense_ft_log(lg, sig64, RPY_F64, &tb, n_streams, tsize, tsize, stream);
+    bool logged = false;
+    if (!rc && dtype != RPY_F64) {
+        // float32 signatures: the float64 log widens them as it loads them;
+        // shapes without a fixed-shape log widen first
+        rc = dense_ft_log_from_f32(lg, (const float*)sig, &tb, n_streams, tsize, tsize, stream);
+        logged = rc == RPY_OK;
+        if (rc == RPY_ERR_UNSUPPORTED) {
+            rc = RPY_OK;
+            hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0, st,
+                               sig64, (const float*)sig, int64_t(tel));
+            if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
+        }
+    }
+    if (!rc && !logged) rc = dense_ft_log(lg, sig64, RPY_F64, &tb, n_streams, tsize, tsize, stream);
     if (!rc) {
         hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(n_streams * lsize)), dim3(256), 0, st,
                            out, n_streams * lsize);

@@ -905,9 +905,10 @@ int launch_inplace(void* lhs, const void* rhs, const DevBasis& b, Bounds lb, Bou
 // a <- a (x) x with x's unit dropped, levels ml..1 top-down.  a and xs hold
 // levels 0..D-1 in LDS; when ml == D the top level's one x term (lhs degree
 // 0) reads the global xg and the top level's outputs go straight to the
-// global o (plus m when given): no product ever reads a's top level.
-template <typename T, int W, int D, int ml>
-RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* xs, T f, const T* __restrict__ xg,
+// global o (plus m when given): no product ever reads a's top level.  xg
+// may be of a narrower type (float32 input of a float64 log).
+template <typename T, int W, int D, int ml, typename Tin>
+RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* xs, T f, const Tin* __restrict__ xg,
                                    T* __restrict__ o, const T* __restrict__ mg)
 {
     chen::static_for<ml>([&](auto mi) {
@@ -926,8 +927,13 @@ RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* xs, T f, const T* __restrict__
                 const uint32_t pi = u / k;
                 const uint32_t si = u - pi * k;
                 const T av = a[dbl + pi];
-                const T* xb = (top && ld == 0 ? xg : xs) + dbr + si * W;
-                chen::static_for<W>([&](auto c) { acc[c] += (av * xb[c]) * f; });
+                if constexpr (top && ld == 0) {
+                    const Tin* xb = xg + dbr + si * W;
+                    chen::static_for<W>([&](auto c) { acc[c] += (av * T(xb[c])) * f; });
+                } else {
+                    const T* xb = xs + dbr + si * W;
+                    chen::static_for<W>([&](auto c) { acc[c] += (av * xb[c]) * f; });
+                }
             });
             if constexpr (top) {
                 T* oo = o + dbm + u * W;
@@ -946,10 +952,10 @@ RPY_FI void wg_mul_by_nounit_fixed(T* a, const T* xs, T f, const T* __restrict__
     __syncthreads();
 }
 
-template <typename T, Horner H, int W, int D>
+template <typename T, Horner H, int W, int D, typename Tin = T>
 __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ out,
                                                              const T* __restrict__ mul,
-                                                             const T* __restrict__ arg,
+                                                             const Tin* __restrict__ arg,
                                                              int64_t batch, int64_t os, int64_t ms,
                                                              int64_t as)
 {
@@ -962,10 +968,10 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
     T* xs = a + np;
     for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
         T* o = out + e * os;
-        const T* x = arg + e * as;
+        const Tin* x = arg + e * as;
         const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
         for (uint32_t i = threadIdx.x; i < ns; i += kBlock) {
-            xs[i] = x[i];
+            xs[i] = T(x[i]);
             if constexpr (H == Horner::FmExp)
                 a[i] = m[i];
             else
@@ -998,7 +1004,7 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
     }
 }
 
-template <typename T, Horner H, int W, int D>
+template <typename T, Horner H, int W, int D, typename Tin = T>
 bool launch_horner_fixed(void* out, const void* mul, const void* arg, int64_t batch, int64_t os,
                          int64_t ms, int64_t as, hipStream_t st)
 {
@@ -1007,8 +1013,9 @@ bool launch_horner_fixed(void* out, const void* mul, const void* arg, int64_t ba
     if constexpr (lds > kLdsBudget) {
         return false;
     } else {
-        hipLaunchKernelGGL((horner_fixed_kernel<T, H, W, D>), dim3(grid_for(batch)), dim3(kBlock),
-                           lds, st, (T*)out, (const T*)mul, (const T*)arg, batch, os, ms, as);
+        hipLaunchKernelGGL((horner_fixed_kernel<T, H, W, D, Tin>), dim3(grid_for(batch)),
+                           dim3(kBlock), lds, st, (T*)out, (const T*)mul, (const Tin*)arg, batch,
+                           os, ms, as);
         return true;
     }
 }
@@ -1017,11 +1024,11 @@ template <typename T, Horner H>
 using HornerFixed = bool (*)(void*, const void*, const void*, int64_t, int64_t, int64_t, int64_t,
                              hipStream_t);
 
-template <typename T, Horner H>
+template <typename T, Horner H, typename Tin = T>
 HornerFixed<T, H> find_horner_fixed(int w, int d)
 {
 #define RPY_F(W, D) \
-    if (w == W && d == D) return &launch_horner_fixed<T, H, W, D>;
+    if (w == W && d == D) return &launch_horner_fixed<T, H, W, D, Tin>;
     RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
     RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
     RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
@@ -1500,6 +1507,23 @@ int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_b
                  int64_t batch, int64_t os, int64_t as, void* stream)
 {
     return horner_entry(Horner::Log, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+}
+
+int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis* basis,
+                          int64_t batch, int64_t os, int64_t as, void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b))) return rc;
+    if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    auto f = find_horner_fixed<double, Horner::Log, float>(b.width, b.depth);
+    if (!f || !f(out, nullptr, arg, batch, os, 0, as, as_stream(stream)))
+        return fail(RPY_ERR_UNSUPPORTED, "no fixed-shape log for width %d depth %d", b.width,
+                    b.depth);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
 }
 
 int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy_dtype dtype,
