@@ -30,13 +30,6 @@ __global__ void widen_kernel(double* __restrict__ out, const float* __restrict__
         out[i] = double(in[i]);
 }
 
-__global__ void zero_kernel(double* __restrict__ out, int64_t n)
-{
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x)
-        out[i] = 0.0;
-}
-
 uint32_t blocks_for(int64_t n)
 {
     int64_t b = (n + 255) / 256;
@@ -293,12 +286,10 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
         }
     }
     if (!rc && !logged) rc = dense_ft_log(lg, sig64, RPY_F64, &tb, n_streams, tsize, tsize, stream);
-    if (!rc) {
-        hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(n_streams * lsize)), dim3(256), 0, st,
-                           out, n_streams * lsize);
-        if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "zero launch failed");
-    }
-    if (!rc) rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize, stream);
+    // the projection writes the output (no clearing pass, no read of it)
+    if (!rc)
+        rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize, stream,
+                              false);
     hipError_t ferr = hipFreeAsync(ws, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
     return rc;

@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void csr_map_kernel(T* __restrict__ out,
                                                         const int64_t* __restrict__ idx,
                                                         const T* __restrict__ val, int64_t rows,
                                                         T scale, int64_t batch, int64_t os,
-                                                        int64_t as)
+                                                        int64_t as, bool accumulate)
 {
     const int64_t total = batch * rows;
     for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total;
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void csr_map_kernel(T* __restrict__ out,
         const T* a = arg + e * as;
         T acc = T(0);
         for (int64_t p = offs[r]; p < offs[r + 1]; ++p) acc += (val[p] * a[idx[p]]) * scale;
-        out[e * os + r] += acc;
+        out[e * os + r] = accumulate ? out[e * os + r] + acc : acc;
     }
 }
 
@@ -383,7 +383,8 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
                                                             const int64_t* __restrict__ idx,
                                                             const T* __restrict__ val,
                                                             int64_t rows, int64_t cols, T scale,
-                                                            int64_t batch, int64_t os, int64_t as)
+                                                            int64_t batch, int64_t os, int64_t as,
+                                                            bool accumulate)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T* s = reinterpret_cast<T*>(smem);
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
             T ov[E], acc[E];
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                ov[q] = q < ne ? out[(e0 + q) * os + r] : T(0);
+                ov[q] = (accumulate && q < ne) ? out[(e0 + q) * os + r] : T(0);
                 acc[q] = T(0);
             }
             const int64_t pe = offs[r + 1];
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
             }
 #pragma unroll
             for (int q = 0; q < E; ++q)
-                if (q < ne) out[(e0 + q) * os + r] = ov[q] + acc[q];
+                if (q < ne) out[(e0 + q) * os + r] = accumulate ? ov[q] + acc[q] : acc[q];
         }
     }
 }
@@ -1527,7 +1528,8 @@ int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis*
 }
 
 int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy_dtype dtype,
-                     double scale, int64_t batch, int64_t os, int64_t as, void* stream)
+                     double scale, int64_t batch, int64_t os, int64_t as, void* stream,
+                     bool accumulate)
 {
     int rc;
     if ((rc = plain_dtype(dtype))) return rc;
@@ -1578,7 +1580,7 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
             }
             hipLaunchKernelGGL(kern, grid, dim3(BL), lds, as_stream(stream), (T*)out,
                                (const T*)arg, m->offsets, m->indices, (const T*)m->data, m->rows,
-                               m->cols, T(scale), batch, os, as);
+                               m->cols, T(scale), batch, os, as, accumulate);
         };
         auto by_e = [&](auto tag) {
             switch (E) {
@@ -1599,11 +1601,12 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(csr_map_kernel<double>, grid, block, 0, as_stream(stream),
                            (double*)out, (const double*)arg, m->offsets, m->indices,
-                           (const double*)m->data, m->rows, scale, batch, os, as);
+                           (const double*)m->data, m->rows, scale, batch, os, as, accumulate);
     else
         hipLaunchKernelGGL(csr_map_kernel<float>, grid, block, 0, as_stream(stream),
                            (float*)out, (const float*)arg, m->offsets, m->indices,
-                           (const float*)m->data, m->rows, float(scale), batch, os, as);
+                           (const float*)m->data, m->rows, float(scale), batch, os, as,
+                           accumulate);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }

@@ -31,7 +31,9 @@ int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_b
 // kernel's loads); RPY_ERR_UNSUPPORTED for shapes without a fixed-shape kernel.
 int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis* basis,
                           int64_t batch, int64_t os, int64_t as, void* stream);
+// out += scale * M arg (accumulate), or out = scale * M arg.
 int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy_dtype dtype,
-                     double scale, int64_t batch, int64_t os, int64_t as, void* stream);
+                     double scale, int64_t batch, int64_t os, int64_t as, void* stream,
+                     bool accumulate = true);
 
 }  // namespace rpy_amd
