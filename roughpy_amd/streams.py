@@ -670,9 +670,9 @@ class LieIncrementStream:
         res = self.resolution if resolution is None else _resolution_arg(resolution)
         depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
         n = len(partition)
-        mids = partition.intermediates()
-        infs = np.array([partition.inf] + mids, dtype=np.float64)
-        sups = np.array(mids + [partition.sup], dtype=np.float64)
+        mids = np.fromiter(partition._mids, dtype=np.float64, count=n - 1)
+        infs = np.concatenate([[float(partition.inf)], mids])
+        sups = np.concatenate([mids, [float(partition.sup)]])
         t = _device.require_gpu()
         # the index buffer is pinned host memory kept between calls: a fresh
         # pageable one costs a page fault per 4 KiB on first write and a
