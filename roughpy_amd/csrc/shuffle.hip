@@ -489,28 +489,46 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
     const uint32_t limit = 1u << n;
     uint32_t S = sb == 0 ? (1u << k) - 1u : unrank_subset(sb, k, n);
     for (uint32_t cnt = 0; cnt < sc && S < limit; ++cnt) {
-        // u letters on the complement of S, lowest position = last letter
-        uint32_t U = 0, ur = u;
-        chen::static_for<D>([&](auto jc) {
-            constexpr int j = int(decltype(jc)::value);
-            if (j < n && !((S >> j) & 1u)) {
-                U += (ur % uint32_t(W)) * cpow<W>(j);
-                ur /= uint32_t(W);
-            }
-        });
-        // weights of v's letters: wp[i] = W^(position of the i-th lowest bit of S)
+        // U: the u letters on the complement of S (lowest position = last
+        // letter); wp[i] = W^(position of the i-th lowest bit of S); for
+        // W = 2^b, M: the bit fields of S's positions above its lowest
+        uint32_t U = 0;
         uint32_t wp[D];
-        {
+        uint32_t M = 0;
+        if constexpr ((W & (W - 1)) == 0) {
+            // a level index holds letter j in bits [j b, j b + b): U is u
+            // with an empty field opened at each position of S, in one pass
+            // over S's k bits
+            constexpr uint32_t b = uint32_t(__builtin_ctz(uint32_t(W)));
+            U = u;
             uint32_t t = S;
             chen::static_for<D>([&](auto ic) {
                 constexpr int i = int(decltype(ic)::value);
                 wp[i] = 0;
                 if (i < k) {
-                    if constexpr ((W & (W - 1)) == 0)
-                        wp[i] = 1u << (uint32_t(__builtin_ctz(t)) *
-                                       uint32_t(__builtin_ctz(uint32_t(W))));
-                    else
-                        wp[i] = wpow_rt<W, D>(__builtin_ctz(t));
+                    const uint32_t sh = uint32_t(__builtin_ctz(t)) * b;
+                    t &= t - 1u;
+                    const uint32_t low = (1u << sh) - 1u;
+                    U = ((U & ~low) << b) | (U & low);
+                    wp[i] = 1u << sh;
+                    if (i > 0) M |= uint32_t(W - 1) << sh;
+                }
+            });
+        } else {
+            uint32_t ur = u;
+            chen::static_for<D>([&](auto jc) {
+                constexpr int j = int(decltype(jc)::value);
+                if (j < n && !((S >> j) & 1u)) {
+                    U += (ur % uint32_t(W)) * cpow<W>(j);
+                    ur /= uint32_t(W);
+                }
+            });
+            uint32_t t = S;
+            chen::static_for<D>([&](auto ic) {
+                constexpr int i = int(decltype(ic)::value);
+                wp[i] = 0;
+                if (i < k) {
+                    wp[i] = wpow_rt<W, D>(__builtin_ctz(t));
                     t &= t - 1u;
                 }
             });
@@ -536,12 +554,6 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
             // of S's positions above its lowest, and vh + 1 is the masked
             // increment ((V | ~M) + 1) & M over those fields
             constexpr uint32_t b = uint32_t(__builtin_ctz(uint32_t(W)));
-            uint32_t M = 0;
-            chen::static_for<D>([&](auto jc) {
-                constexpr int j = int(decltype(jc)::value);
-                if (j < n && ((S >> j) & 1u)) M |= uint32_t(W - 1) << (uint32_t(j) * b);
-            });
-            M &= ~(uint32_t(W - 1) << (uint32_t(__builtin_ctz(S)) * b));
             uint32_t ob = dbk + vb * W;
             // (float64 at W >= 4 measured slower unrolled: (4,6) 6.87 -> 7.15 ms)
             constexpr bool kTwo = sizeof(T) == 4 || W * W <= 4;
