@@ -358,7 +358,9 @@ int rpy_stream_select_rows(const double *params, int64_t n_params,
  * rpy_stream_select_rows.  offsets (host, n_queries + 1) receives the CSR
  * offsets of the selected row indices; rows (host, max_rows entries) receives
  * the indices, or may be NULL to size the output first.  The params are
- * checked once per call. */
+ * checked once per call.  Large batches run in blocks of queries on up to 16
+ * host threads, joined before the call returns; the result equals a single
+ * pass in query order. */
 int rpy_stream_select_rows_batch(const double *params, int64_t n_params,
                                  int32_t stream_resolution, double support_inf,
                                  double support_sup, const double *infs,
