@@ -203,7 +203,7 @@ def antipode(a: FreeTensor, **kwargs) -> FreeTensor:
     code = _dtype_code(dt)
     size = a.basis.size()
     arg = _Operand(a.data, dt, size, "arg")
-    res = _new_like((arg.batch, size), dt)
+    res = _new_like((arg.batch, size), dt, zero=False)  # the kernel writes every coefficient
     _capi.check(_capi.lib().rpy_dense_ft_antipode(
         res.data_ptr(), arg.ptr, code, a.basis.c_struct(), arg.batch, size, arg.stride,
         _device.stream_handle()))
@@ -220,7 +220,7 @@ def _unary_out(x, out_basis, entry, name):
     code = _dtype_code(dt, ValueError)
     size = out_basis.size()
     arg = _padded(x.data, size, dt, name)
-    res = _new_like((arg.batch, size), dt)
+    res = _new_like((arg.batch, size), dt, zero=False)  # exp and log write every coefficient
     _capi.check(entry(res.data_ptr(), arg.ptr, code, out_basis.c_struct(), arg.batch, size,
                       arg.stride, _device.stream_handle()))
     res = res.reshape(_shape(x.data)[:-1] + (size,))
@@ -250,7 +250,7 @@ def ft_fmexp(multiplier: FreeTensor, exponent: FreeTensor,
     size = out_basis.size()
     mul = _padded(multiplier.data, size, dt, "multiplier")
     exp = _padded(exponent.data, size, dt, "exponent")
-    res = _new_like((mul.batch, size), dt)
+    res = _new_like((mul.batch, size), dt, zero=False)  # the kernel writes every coefficient
     _capi.check(_capi.lib().rpy_dense_ft_fmexp(
         res.data_ptr(), mul.ptr, exp.ptr, code, out_basis.c_struct(), mul.batch, size,
         mul.stride, exp.stride, _device.stream_handle()))
