@@ -375,7 +375,10 @@ int rpy_stream_select_rows_batch(const double* params, int64_t n_params,
         const auto [q0, q1] = bounds(t);
         return select_block(args, q0, q1, nullptr, rows + offsets[q0]);
     });
-    return bad >= 0 ? RPY_ERR_INVALID_ARGUMENT : RPY_OK;
+    // the filling pass repeats the counting pass's work, so it fails only if
+    // that did
+    if (bad >= 0) return fail(RPY_ERR_INVALID_ARGUMENT, "row selection failed while filling");
+    return RPY_OK;
 }
 
 }  // extern "C"
