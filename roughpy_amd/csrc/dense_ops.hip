@@ -812,6 +812,32 @@ void launch_inplace_fixed(void* lhs, const void* rhs, int64_t batch, int64_t ls,
 
 using InplaceFixed = void (*)(void*, const void*, int64_t, int64_t, int64_t, hipStream_t);
 
+// out = lhs (x) rhs at full depth, out not read (a buffer of its own)
+template <typename T, int W, int D>
+void launch_mul_fixed(void* out, const void* lhs, const void* rhs, int64_t batch, int64_t os,
+                      int64_t ls, int64_t rs, hipStream_t st)
+{
+    hipLaunchKernelGGL((ft_fma_fixed_kernel<T, T, W, D, false>), dim3(grid_for(batch)),
+                       dim3(kBlock), 0, st, (T*)out, (const T*)lhs, (const T*)rhs, batch, os, ls,
+                       rs);
+}
+
+template <typename T>
+FmaFixed find_mul_fixed(int w, int d)
+{
+#define RPY_F(W, D) \
+    if (w == W && d == D) return &launch_mul_fixed<T, W, D>;
+    RPY_F(2, 2) RPY_F(2, 3) RPY_F(2, 4) RPY_F(2, 5) RPY_F(2, 6) RPY_F(2, 7) RPY_F(2, 8)
+    RPY_F(3, 2) RPY_F(3, 3) RPY_F(3, 4) RPY_F(3, 5) RPY_F(3, 6)
+    RPY_F(4, 2) RPY_F(4, 3) RPY_F(4, 4) RPY_F(4, 5) RPY_F(4, 6)
+    RPY_F(5, 2) RPY_F(5, 3) RPY_F(5, 4) RPY_F(5, 5)
+    RPY_F(6, 2) RPY_F(6, 3) RPY_F(6, 4) RPY_F(6, 5)
+    RPY_F(7, 2) RPY_F(7, 3) RPY_F(7, 4)
+    RPY_F(8, 2) RPY_F(8, 3) RPY_F(8, 4)
+#undef RPY_F
+    return nullptr;
+}
+
 template <typename T>
 InplaceFixed find_inplace_fixed(int w, int d)
 {
@@ -1508,6 +1534,26 @@ int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_b
                  int64_t batch, int64_t os, int64_t as, void* stream)
 {
     return horner_entry(Horner::Log, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+}
+
+int dense_ft_mul_new(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ls,
+                     int64_t rs, void* stream)
+{
+    DevBasis b;
+    int rc;
+    if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    if ((rc = check_batch(batch, {os, ls, rs}, b.size()))) return rc;
+    if (batch == 0) return RPY_OK;
+    if (!out || !lhs || !rhs) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    const FmaFixed f = dtype == RPY_F64 ? find_mul_fixed<double>(b.width, b.depth)
+                                        : find_mul_fixed<float>(b.width, b.depth);
+    if (!f)
+        return fail(RPY_ERR_UNSUPPORTED, "no fixed-shape product for width %d depth %d",
+                    b.width, b.depth);
+    f(out, lhs, rhs, batch, os, ls, rs, as_stream(stream));
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
 }
 
 int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis* basis,

@@ -27,6 +27,12 @@ int dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype
                    int64_t es, void* stream);
 int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
                  int64_t batch, int64_t os, int64_t as, void* stream);
+// out = lhs (x) rhs at full depth into a buffer of its own (out is written,
+// not read); RPY_ERR_UNSUPPORTED for shapes without a fixed-shape kernel.
+int dense_ft_mul_new(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ls,
+                     int64_t rs, void* stream);
+
 // The float64 tensor log of float32 tensors (the widening fused into the
 // kernel's loads); RPY_ERR_UNSUPPORTED for shapes without a fixed-shape kernel.
 int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis* basis,
