@@ -622,7 +622,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(the streams shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the streams shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import torch
