@@ -993,16 +993,20 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* xs = a + np;
+    T* mls = xs + np;  // fmexp: levels 0..D-1 of the multiplier, added after every step
     for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
         T* o = out + e * os;
         const Tin* x = arg + e * as;
         const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
         for (uint32_t i = threadIdx.x; i < ns; i += kBlock) {
             xs[i] = T(x[i]);
-            if constexpr (H == Horner::FmExp)
-                a[i] = m[i];
-            else
+            if constexpr (H == Horner::FmExp) {
+                const T mv = m[i];
+                a[i] = mv;
+                mls[i] = mv;
+            } else {
                 a[i] = (H == Horner::Exp && i == 0) ? T(1) : T(0);
+            }
         }
         __syncthreads();
         chen::static_for<D>([&](auto di) {
@@ -1021,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
                 if constexpr (H == Horner::Exp) {
                     if (threadIdx.x == 0) a[0] += T(1);
                 } else {
-                    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) a[i] += m[i];
+                    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) a[i] += mls[i];
                 }
                 __syncthreads();
             }
@@ -1036,7 +1040,7 @@ bool launch_horner_fixed(void* out, const void* mul, const void* arg, int64_t ba
                          int64_t ms, int64_t as, hipStream_t st)
 {
     constexpr size_t np = (size_t(fdb<W>(D)) + 15) / 16 * 16;
-    constexpr size_t lds = 2 * np * sizeof(T);
+    constexpr size_t lds = (H == Horner::FmExp ? 3 : 2) * np * sizeof(T);
     if constexpr (lds > kLdsBudget) {
         return false;
     } else {
