@@ -998,16 +998,31 @@ __global__ __launch_bounds__(kBlock) void horner_fixed_kernel(T* __restrict__ ou
         T* o = out + e * os;
         const Tin* x = arg + e * as;
         const T* m = (H == Horner::FmExp) ? mul + e * ms : nullptr;
-        for (uint32_t i = threadIdx.x; i < ns; i += kBlock) {
-            xs[i] = T(x[i]);
-            if constexpr (H == Horner::FmExp) {
-                const T mv = m[i];
-                a[i] = mv;
-                mls[i] = mv;
-            } else {
-                a[i] = (H == Horner::Exp && i == 0) ? T(1) : T(0);
+        // every staging load of the tensor goes out before its LDS stores
+        constexpr int kIt = int((ns + kBlock - 1) / kBlock);
+        Tin xv[kIt];
+        T mv[H == Horner::FmExp ? kIt : 1];
+        chen::static_for<kIt>([&](auto k0) {
+            constexpr int k = int(decltype(k0)::value);
+            const uint32_t i = threadIdx.x + uint32_t(k) * kBlock;
+            if (i < ns) {
+                xv[k] = x[i];
+                if constexpr (H == Horner::FmExp) mv[k] = m[i];
             }
-        }
+        });
+        chen::static_for<kIt>([&](auto k0) {
+            constexpr int k = int(decltype(k0)::value);
+            const uint32_t i = threadIdx.x + uint32_t(k) * kBlock;
+            if (i < ns) {
+                xs[i] = T(xv[k]);
+                if constexpr (H == Horner::FmExp) {
+                    a[i] = mv[k];
+                    mls[i] = mv[k];
+                } else {
+                    a[i] = (H == Horner::Exp && i == 0) ? T(1) : T(0);
+                }
+            }
+        });
         __syncthreads();
         chen::static_for<D>([&](auto di) {
             constexpr int deg = D - int(decltype(di)::value);
