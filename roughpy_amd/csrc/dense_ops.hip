@@ -743,10 +743,30 @@ __global__ __launch_bounds__(kBlock) void ft_fma_fixed_kernel(T* out, const T* l
             }
         };
         load_top(threadIdx.x);
-        for (int i = threadIdx.x; i < kStaged; i += kBlock) {
-            sl[i] = Acc(l[i]);
-            sr[i] = Acc(r[i]);
-        }
+        // staging loads in groups of up to 8 per operand, each group's loads
+        // issued before its LDS stores
+        constexpr int kSIt = (kStaged + kBlock - 1) / kBlock;
+        constexpr int kSG = kSIt < 8 ? kSIt : 8;
+        chen::static_for<(kSIt + kSG - 1) / kSG>([&](auto g0) {
+            constexpr int g = int(decltype(g0)::value);
+            T lv[kSG], rv[kSG];
+            chen::static_for<kSG>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + (g * kSG + k) * kBlock;
+                if (g * kSG + k < kSIt && i < kStaged) {
+                    lv[k] = l[i];
+                    rv[k] = r[i];
+                }
+            });
+            chen::static_for<kSG>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const int i = threadIdx.x + (g * kSG + k) * kBlock;
+                if (g * kSG + k < kSIt && i < kStaged) {
+                    sl[i] = Acc(lv[k]);
+                    sr[i] = Acc(rv[k]);
+                }
+            });
+        });
         __syncthreads();
         for (int base = threadIdx.x; base < kN; base += kTopU * kBlock) {
             if (base != int(threadIdx.x)) load_top(base);
