@@ -379,13 +379,40 @@ __global__ __launch_bounds__(kBlock) void dyadic_query_fast_kernel(
     auto run = [&](chen::Lane<T, W, D, P>& st, int64_t r0, int64_t r1) {
         for (int64_t cs = r0; cs < r1; cs += kQueryChunk) {
             const int n = int(min<int64_t>(kQueryChunk, r1 - cs));
-            for (int e = threadIdx.x; e < n * kDW; e += kBlock) {
-                const int row = e / kDW;
-                const int da = e - row * kDW;
-                const int d = da / W + 1;
-                const int a = da - (d - 1) * W;
-                const T x = incr[(cs + row) * W + a];
-                XT[e] = (d == 1) ? x : x * (T(1) / T(d));
+            if constexpr (sizeof(T) == 8) {
+                // float64: all of a thread's loads of the chunk go out before
+                // its stores (7.28 -> 5.19 ms for 100k queries; the same in
+                // float32 measured slower, 2.52 -> 2.85 ms)
+                constexpr int kIt = (kQueryChunk * kDW + kBlock - 1) / kBlock;
+                T xv[kIt];
+                chen::static_for<kIt>([&](auto k0) {
+                    constexpr int k = int(decltype(k0)::value);
+                    const int e = int(threadIdx.x) + k * kBlock;
+                    if (e < n * kDW) {
+                        const int row = e / kDW;
+                        const int da = e - row * kDW;
+                        const int a = da - (da / W) * W;
+                        xv[k] = incr[(cs + row) * W + a];
+                    }
+                });
+                chen::static_for<kIt>([&](auto k0) {
+                    constexpr int k = int(decltype(k0)::value);
+                    const int e = int(threadIdx.x) + k * kBlock;
+                    if (e < n * kDW) {
+                        const int da = e - (e / kDW) * kDW;
+                        const int d = da / W + 1;
+                        XT[e] = (d == 1) ? xv[k] : xv[k] * (T(1) / T(d));
+                    }
+                });
+            } else {
+                for (int e = threadIdx.x; e < n * kDW; e += kBlock) {
+                    const int row = e / kDW;
+                    const int da = e - row * kDW;
+                    const int d = da / W + 1;
+                    const int a = da - (d - 1) * W;
+                    const T x = incr[(cs + row) * W + a];
+                    XT[e] = (d == 1) ? x : x * (T(1) / T(d));
+                }
             }
             __syncthreads();
             if (active)
