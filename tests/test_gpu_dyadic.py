@@ -23,8 +23,11 @@ def _streams(rng, n, w, lmin, lmax, dtype):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
-def test_cache_nodes_and_queries(gpu, rng, dtype, tol):
-    w, d, r, J = 3, 4, 8, 3
+@pytest.mark.parametrize("w,d", [(3, 4), (2, 9)])
+def test_cache_nodes_and_queries(gpu, rng, dtype, tol, w, d):
+    """(3, 4) builds parent nodes with the fixed-shape product that writes
+    them; (2, 9) has no fixed-shape kernel and clears them, then adds."""
+    r, J = 8, 3
     inc, params, offs, lens = _streams(rng, 7, w, 0, 300, dtype)
     cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J, dtype=dtype)
     inc64 = inc.astype(np.float64)
