@@ -86,6 +86,19 @@ def test_log_signature_batch_vs_oracle(gpu, rng):
             assert_allclose(got[s], ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
+@pytest.mark.parametrize("w,d", [(5, 5), (4, 6), (2, 9)])
+def test_log_signature_batch_float32_vs_oracle(gpu, rng, w, d):
+    """float32 increments: float32 signatures, then the float64 log (which
+    widens them as it loads them at (5,5) and (4,6); (2,9) has no fixed-shape
+    log and widens first) and the t2l projection, which writes its output."""
+    inc, offs = _uneven(rng, 4, 1, 60, w, np.float64)
+    inc32 = (inc / 2).astype(np.float32)
+    got = rs.log_signature_batch(inc32, offs, w, d)
+    for s in range(4):
+        ref = O.log_signature(inc32[offs[s]:offs[s + 1]].astype(np.float64), w, d)
+        assert_allclose(got[s], ref, rtol=0, atol=2e-5 * max(1.0, np.abs(ref).max()))
+
+
 def test_brownian_w3_d4_1000_increments(gpu):
     """One stream, width 3, 1000 increments ~ N(0, 1/1000), depth 4: signature
     (121) and log-signature (32) through LieIncrementStream."""
