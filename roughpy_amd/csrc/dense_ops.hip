@@ -428,12 +428,20 @@ __global__ __launch_bounds__(Block) void csr_map_lds_kernel(T* __restrict__ out,
             }
         }
         __syncthreads();
+        // the output values a row adds to are loaded one row-pass ahead
+        T ovn[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q)
+            ovn[q] = (accumulate && q < ne && int64_t(threadIdx.x) < rows)
+                             ? out[(e0 + q) * os + threadIdx.x]
+                             : T(0);
         for (int64_t r = threadIdx.x; r < rows; r += Block) {
-            // the output values this row adds to load while its entries run
             T ov[E], acc[E];
+            const int64_t rn = r + Block;
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                ov[q] = (accumulate && q < ne) ? out[(e0 + q) * os + r] : T(0);
+                ov[q] = ovn[q];
+                ovn[q] = (accumulate && q < ne && rn < rows) ? out[(e0 + q) * os + rn] : T(0);
                 acc[q] = T(0);
             }
             const int64_t pe = offs[r + 1];
