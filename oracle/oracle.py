@@ -228,8 +228,10 @@ def dynkin_word(width: int, word: Sequence[int]) -> np.ndarray:
     return r
 
 
+@lru_cache(maxsize=None)
 def t2l_dense(width: int, depth: int) -> np.ndarray:
-    """Dense t2l matrix (lie_dim x tensor_dim).
+    """Dense t2l matrix (lie_dim x tensor_dim), computed once per shape and
+    returned read-only.
 
     Column of word u (degree k) is r(u)/k expanded in the Hall basis, the
     definition used by lie_basis.c:1150-1250 (rbracketing then 1/deg,
@@ -262,6 +264,7 @@ def t2l_dense(width: int, depth: int) -> np.ndarray:
         if np.abs(C - Ci).max() > 1e-8:
             raise AssertionError("non-integral Hall expansion")
         out[np.array(keys) - 1, db_t[k]:db_t[k + 1]] = Ci / k
+    out.flags.writeable = False
     return out
 
 

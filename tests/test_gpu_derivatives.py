@@ -211,3 +211,53 @@ def test_signature_vjp_kernels_vs_host_lane_algebra(gpu, rng, w, d):
         ref64 = got[offs[s]:offs[s + 1]]
         scale = max(1.0, np.abs(ref64).max())
         assert np.abs(got32[offs[s]:offs[s + 1]] - ref64).max() <= 1e-4 * scale
+
+
+def _log_loss_oracle(seg, g, w, d):
+    return float(np.dot(O.log_signature(seg, w, d), g))
+
+
+def test_log_signature_vjp_w8_d4_4096_streams(gpu):
+    """Log-signatures at width 8, depth 4 (Lie dimension 1212), 4096 streams of
+    length 256 with N(0, 1/256) increments; loss <G, logsig> with a fixed
+    seeded G.  Float64 gradient vs float64 central differences (step 1e-6) of
+    the oracle log-signature on 64 sampled increments; the float32 gradient's
+    error against the float64 one stated."""
+    torch = gpu
+    rng = np.random.default_rng(12345 + 5)
+    w, d, n, L = 8, 4, 4096, 256
+    inc = rng.normal(0.0, np.sqrt(1.0 / L), size=(n * L, w))
+    offs = np.arange(0, n * L + 1, L, dtype=np.int64)
+    lsize = sum(O.witt_dim(w, k) for k in range(1, d + 1))
+    Gl = rng.normal(size=(n, lsize))
+    to = torch.from_numpy(offs).cuda()
+    x64 = torch.from_numpy(inc).cuda().requires_grad_(True)
+    ls64 = ag.log_signature(x64, to, w, d)
+    pick = rng.choice(n, size=4, replace=False)
+    for s in pick:
+        ref = O.log_signature(inc[offs[s]:offs[s + 1]], w, d)
+        assert_allclose(ls64[s].detach().cpu().numpy(), ref, rtol=0,
+                        atol=1e-12 * max(1.0, np.abs(ref).max()))
+    (ls64 * torch.from_numpy(Gl).cuda()).sum().backward()
+    g64 = x64.grad.cpu().numpy()
+    x32 = torch.from_numpy(inc.astype(np.float32)).cuda().requires_grad_(True)
+    (ag.log_signature(x32, to, w, d) * torch.from_numpy(Gl).cuda()).sum().backward()
+    g32 = x32.grad.cpu().numpy().astype(np.float64)
+    err32 = np.abs(g32 - g64).max() / np.abs(g64).max()
+    print(f"log-signature w8 d4 4096x256: float32 gradient max error / max |grad| = {err32:.2e}")
+    assert err32 < 1e-3
+    h = 1e-6
+    worst = 0.0
+    for _ in range(64):
+        s = int(rng.integers(0, n))
+        r = int(rng.integers(0, L))
+        a = int(rng.integers(0, w))
+        ip = inc[offs[s]:offs[s + 1]].copy()
+        im = ip.copy()
+        ip[r, a] += h
+        im[r, a] -= h
+        fd = (_log_loss_oracle(ip, Gl[s], w, d) - _log_loss_oracle(im, Gl[s], w, d)) / (2 * h)
+        worst = max(worst, abs(g64[offs[s] + r, a] - fd) / max(1.0, abs(fd)))
+    print(f"log-signature w8 d4 4096x256: float64 gradient vs central differences, "
+          f"worst {worst:.2e}")
+    assert worst < 1e-6

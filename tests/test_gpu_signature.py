@@ -3,10 +3,12 @@
 float64: within 1e-12 normwise per level of the oracle on the same inputs.
 float32: error against the float64 oracle on the same float32 inputs is
 stated and bounded (2e-5 here, lengths up to 1000).
-At the full benchmark size (16384 streams, lengths up to 4096) the checks are
-size-independent properties: level one equals the sum of increments, Chen's
-identity between two halves of a stream, agreement between the float32 and
-float64 kernels, and a sampled comparison with the oracle.
+At the full benchmark size (16384 streams, lengths up to 4096) float32
+signatures are held to 1e-5 (max normwise relative error per level) against
+the float64 oracle on the 32 longest and 32 random streams, and on every
+stream through the float64 kernel, which equals the oracle there within
+1e-12; the other checks are size-independent properties: level one equals
+the sum of increments and Chen's identity between two halves of a stream.
 """
 import json
 import os
@@ -174,15 +176,25 @@ def test_benchmark_size_properties(gpu):
     # level one = sum of increments (float64 kernel)
     sums = np.add.reduceat(inc.astype(np.float64), offs[:-1], axis=0)
     assert_allclose(a64[:, 1:1 + w], sums, rtol=1e-12, atol=1e-12)
-    # float32 kernel agrees with the float64 kernel
+    # float32 signatures within 1e-5 (max normwise relative error per level)
+    # of the float64 oracle: the 32 longest streams and 32 random ones against
+    # the oracle itself, on the float32 increments promoted to float64 ...
+    longest = np.argsort(-lens, kind="stable")[:32]
+    pick = np.concatenate([longest, rng.choice(n, size=32, replace=False)])
+    sub_lens = lens[pick]
+    sub_offs = np.concatenate([[0], np.cumsum(sub_lens)]).astype(np.int64)
+    sub_inc = np.concatenate([inc[offs[s]:offs[s + 1]] for s in pick]).astype(np.float64)
+    ref = O.signature_batch(sub_inc, sub_offs, w, d, nthreads=8)
+    err_oracle = O.level_errors(a32[pick], ref, w, d).max()
+    print(f"benchmark workload f32 vs f64 oracle (32 longest + 32 random streams): "
+          f"{err_oracle:.3e}")
+    assert err_oracle <= 1e-5
+    # ... the float64 kernel equals the oracle there within 1e-12 ...
+    assert O.level_errors(a64[pick], ref, w, d).max() <= 1e-12
+    # ... so all 16384 streams are held to the same bar through it
     err32 = O.level_errors(a32, a64, w, d).max()
-    print(f"benchmark workload f32 vs f64 kernel max normwise level error: {err32:.3e}")
-    assert err32 < 5e-5
-    # sampled exact comparison with the oracle
-    pick = rng.choice(n, size=6, replace=False)
-    for s in pick:
-        ref = O.signature(inc[offs[s]:offs[s + 1]].astype(np.float64), w, d)
-        assert O.level_errors(a64[s], ref, w, d).max() <= 1e-12
+    print(f"benchmark workload f32 vs f64 kernel, all streams: {err32:.3e}")
+    assert err32 <= 1e-5
     # Chen identity: split every stream in two and multiply the halves
     mid = offs[:-1] + lens // 2
     offs2 = np.empty(2 * n + 1, dtype=np.int64)
