@@ -129,7 +129,25 @@ int rpy_t2l_matrix(const rpy_lie_basis *basis, int64_t *nnz, int64_t *offsets,
 
 /* ---- dense free-tensor operations (device) ------------------------------ */
 
-/* out += lhs (x) rhs, truncated; replaces dense_ft_fma
+/* Operand layout of the dense operations.  Element e of operand X is read
+ * and written at X + e*X_stride, and its coefficient i at
+ * X + e*X_stride + i*X_inc (all in elements).  X_inc is the element stride
+ * of the last axis: 1 for contiguous tensors, anything nonzero otherwise
+ * (the reference walks such arrays with StridedDenseIterator,
+ * roughpy_compute/dense/views.hpp:23-170, chosen in
+ * roughpy/compute/_src/py_ternary_array_fn.hpp:41-62).  An output's
+ * elements must not overlap.  Operands with X_inc != 1 are gathered into a
+ * contiguous workspace copy and scattered back; X_inc == 1 takes no copy.
+ *
+ * Degree bounds (rpy_degree_bounds) follow update_algebra_params
+ * (roughpy/compute/_src/call_config.cpp:11-31): max_degree -1 or above the
+ * basis depth means the depth; min_degree above the resolved max_degree is
+ * RPY_ERR_INVALID_ARGUMENT.  An input's coefficients outside its bounds are
+ * treated as zero; an output's coefficients outside its bounds are left
+ * unchanged.  Every tensor operand holds basis size coefficients. */
+
+/* out += lhs (x) rhs, truncated; replaces dense_ft_fma(out, lhs, rhs,
+ * basis, out_depth, lhs_depth, rhs_depth)
  * (roughpy/compute/_src/dense_basic.cpp:85-151, kernel
  * roughpy_compute/dense/basic/free_tensor_fma.hpp:13-53).  dtype RPY_F32,
  * RPY_F64, or RPY_F32_COMPENSATED (float data, each output coefficient
@@ -139,42 +157,58 @@ int rpy_dense_ft_fma(void *out, const void *lhs, const void *rhs,
                      rpy_degree_bounds out_bounds, rpy_degree_bounds lhs_bounds,
                      rpy_degree_bounds rhs_bounds, int64_t batch,
                      int64_t out_stride, int64_t lhs_stride,
-                     int64_t rhs_stride, void *stream);
+                     int64_t rhs_stride, int64_t out_inc, int64_t lhs_inc,
+                     int64_t rhs_inc, void *stream);
 
-/* lhs <- lhs (x) rhs; replaces dense_ft_inplace_mul (dense_basic.cpp:
- * 195-260, kernel free_tensor_inplace_mul.hpp:13-87). */
+/* lhs <- lhs (x) rhs; replaces dense_ft_inplace_mul(lhs, rhs, basis,
+ * out_depth, rhs_depth) (dense_basic.cpp:195-260, kernel
+ * free_tensor_inplace_mul.hpp:13-87). */
 int rpy_dense_ft_inplace_mul(void *lhs, const void *rhs, rpy_dtype dtype,
                              const rpy_tensor_basis *basis,
                              rpy_degree_bounds lhs_bounds,
                              rpy_degree_bounds rhs_bounds, int64_t batch,
                              int64_t lhs_stride, int64_t rhs_stride,
-                             void *stream);
+                             int64_t lhs_inc, int64_t rhs_inc, void *stream);
 
-/* out = antipode(arg); replaces dense_ft_antipode (dense_basic.cpp:
- * 295-360, kernel free_tensor_antipode.hpp:45-100). */
+/* out = antipode(arg) on the levels max(mins)..min(maxes) of the two
+ * bounds (and level 1 whenever that range starts at or below 1, as
+ * free_tensor_antipode.hpp:60-80 does); replaces dense_ft_antipode(out, arg,
+ * basis, out_depth) (dense_basic.cpp:295-360, kernel
+ * free_tensor_antipode.hpp:45-100), whose one depth argument bounds the
+ * argument (dense_basic.cpp:310-318). */
 int rpy_dense_ft_antipode(void *out, const void *arg, rpy_dtype dtype,
-                          const rpy_tensor_basis *basis, int64_t batch,
+                          const rpy_tensor_basis *basis,
+                          rpy_degree_bounds out_bounds,
+                          rpy_degree_bounds arg_bounds, int64_t batch,
                           int64_t out_stride, int64_t arg_stride,
-                          void *stream);
+                          int64_t out_inc, int64_t arg_inc, void *stream);
 
 /* out += L_op^*(arg) (adjoint of left multiplication); replaces
- * dense_ft_adjoint_left_mul (roughpy/compute/_src/dense_basic.cpp:387-490,
- * kernel free_tensor_adjoint_left_mul.hpp:10-77). */
+ * dense_ft_adjoint_left_mul(out, op, arg, basis, out_depth, lhs_depth,
+ * rhs_depth) (roughpy/compute/_src/dense_basic.cpp:387-490, kernel
+ * free_tensor_adjoint_left_mul.hpp:10-77), with that kernel's degree ranges:
+ * argument degrees from max(arg min - op max, out min) to
+ * min(arg max - op min, out max). */
 int rpy_dense_ft_adjoint_left_mul(void *out, const void *op, const void *arg,
                                   rpy_dtype dtype,
                                   const rpy_tensor_basis *basis,
+                                  rpy_degree_bounds out_bounds,
+                                  rpy_degree_bounds op_bounds,
+                                  rpy_degree_bounds arg_bounds,
                                   int64_t batch, int64_t out_stride,
                                   int64_t op_stride, int64_t arg_stride,
-                                  void *stream);
+                                  int64_t out_inc, int64_t op_inc,
+                                  int64_t arg_inc, void *stream);
 
-/* out += lhs (shuffle) rhs at full depth (1..12); replaces dense_st_fma
- * (roughpy/compute/_src/dense_basic.cpp:449-534, kernel
+/* out += lhs (shuffle) rhs at full depth (1..12); replaces dense_st_fma(out,
+ * lhs, rhs, basis) (roughpy/compute/_src/dense_basic.cpp:449-534, kernel
  * roughpy_compute/dense/basic/shuffle_tensor_product.hpp:24-85).  Same
  * mask order as the reference, so float results round the same way. */
 int rpy_dense_st_fma(void *out, const void *lhs, const void *rhs,
                      rpy_dtype dtype, const rpy_tensor_basis *basis,
                      int64_t batch, int64_t out_stride, int64_t lhs_stride,
-                     int64_t rhs_stride, void *stream);
+                     int64_t rhs_stride, int64_t out_inc, int64_t lhs_inc,
+                     int64_t rhs_inc, void *stream);
 
 /* out += adjoint of shuffle multiplication by op, applied to arg:
  * <out, x> = <arg, op (shuffle) x>; the kernel of
@@ -183,28 +217,47 @@ int rpy_dense_st_fma(void *out, const void *lhs, const void *rhs,
 int rpy_dense_st_adj_mul(void *out, const void *op, const void *arg,
                          rpy_dtype dtype, const rpy_tensor_basis *basis,
                          int64_t batch, int64_t out_stride, int64_t op_stride,
-                         int64_t arg_stride, void *stream);
+                         int64_t arg_stride, int64_t out_inc, int64_t op_inc,
+                         int64_t arg_inc, void *stream);
 
-/* out = exp(arg) (arg[0] ignored); replaces dense_ft_exp
+/* out = exp(arg) (arg[0] ignored) truncated at out_bounds.max_degree m:
+ * coefficients 0 .. db[m+1]-1 of out are written; replaces
+ * dense_ft_exp(out, arg, basis, out_depth, arg_depth)
  * (roughpy/compute/_src/dense_intermediate.cpp:62-102, kernel
- * free_tensor_exp.hpp:13-37). */
+ * free_tensor_exp.hpp:13-37).  out_bounds.min_degree must be 0.  The
+ * reference reads both views with out_depth's bounds (:38-52), so a drop-in
+ * binding passes out_depth for arg_bounds too. */
 int rpy_dense_ft_exp(void *out, const void *arg, rpy_dtype dtype,
-                     const rpy_tensor_basis *basis, int64_t batch,
-                     int64_t out_stride, int64_t arg_stride, void *stream);
+                     const rpy_tensor_basis *basis,
+                     rpy_degree_bounds out_bounds, rpy_degree_bounds arg_bounds,
+                     int64_t batch, int64_t out_stride, int64_t arg_stride,
+                     int64_t out_inc, int64_t arg_inc, void *stream);
 
-/* out = multiplier * exp(exponent) (exponent[0] ignored); replaces
- * dense_ft_fmexp (dense_intermediate.cpp:153-208, kernel
- * free_tensor_fmexp.hpp:15-40). */
+/* out = multiplier * exp(exponent) (exponent[0] ignored) truncated at
+ * out_bounds.max_degree; replaces dense_ft_fmexp(out, multiplier, exponent,
+ * basis, out_depth, multiplier_depth, exponent_depth)
+ * (dense_intermediate.cpp:153-208, kernel free_tensor_fmexp.hpp:15-40),
+ * which reads all three views with out_depth's bounds (:124-144): a drop-in
+ * binding passes out_depth for all three. */
 int rpy_dense_ft_fmexp(void *out, const void *multiplier, const void *exponent,
                        rpy_dtype dtype, const rpy_tensor_basis *basis,
-                       int64_t batch, int64_t out_stride, int64_t mul_stride,
-                       int64_t exp_stride, void *stream);
+                       rpy_degree_bounds out_bounds,
+                       rpy_degree_bounds multiplier_bounds,
+                       rpy_degree_bounds exponent_bounds, int64_t batch,
+                       int64_t out_stride, int64_t mul_stride,
+                       int64_t exp_stride, int64_t out_inc, int64_t mul_inc,
+                       int64_t exp_inc, void *stream);
 
-/* out = log(arg), assuming arg[0] == 1; replaces dense_ft_log
- * (dense_intermediate.cpp:254-295, kernel free_tensor_log.hpp:12-40). */
+/* out = log(arg), assuming arg[0] == 1, truncated at out_bounds.max_degree,
+ * with arg read within arg_bounds; replaces dense_ft_log(out, arg, basis,
+ * out_depth, arg_depth) (dense_intermediate.cpp:254-295, kernel
+ * free_tensor_log.hpp:12-40).  out is write-only (the reference's Python
+ * wrapper zero-fills it, roughpy/compute/__init__.py:279). */
 int rpy_dense_ft_log(void *out, const void *arg, rpy_dtype dtype,
-                     const rpy_tensor_basis *basis, int64_t batch,
-                     int64_t out_stride, int64_t arg_stride, void *stream);
+                     const rpy_tensor_basis *basis,
+                     rpy_degree_bounds out_bounds, rpy_degree_bounds arg_bounds,
+                     int64_t batch, int64_t out_stride, int64_t arg_stride,
+                     int64_t out_inc, int64_t arg_inc, void *stream);
 
 /* out[e] += scale * M arg[e] for a device CSR/CSC matrix; replaces
  * dense_lie_to_tensor / dense_tensor_to_lie (dense_basic.cpp:862-1047,
@@ -213,7 +266,8 @@ int rpy_dense_ft_log(void *out, const void *arg, rpy_dtype dtype,
 int rpy_dense_sparse_map(void *out, const void *arg,
                          const rpy_sparse_matrix *matrix, rpy_dtype dtype,
                          double scale, int64_t batch, int64_t out_stride,
-                         int64_t arg_stride, void *stream);
+                         int64_t arg_stride, int64_t out_inc, int64_t arg_inc,
+                         void *stream);
 
 /* ---- streams (device) --------------------------------------------------- */
 
@@ -308,7 +362,8 @@ int rpy_dense_ft_adjoint_right_mul(void *out, const void *op, const void *arg,
                                    const rpy_tensor_basis *basis,
                                    int64_t batch, int64_t out_stride,
                                    int64_t op_stride, int64_t arg_stride,
-                                   void *stream);
+                                   int64_t out_inc, int64_t op_inc,
+                                   int64_t arg_inc, void *stream);
 
 /* ---- signature kernel by signature dot (device) ------------------------- */
 

@@ -465,6 +465,15 @@ def param_to_resolution(arg: float) -> int:
     return -min(0, e - 1)
 
 
+def py_param_to_resolution(arg: float) -> int:
+    """The Python bindings' conversion, which caps at 5 instead of 0
+    (roughpy/src/args/kwargs_to_path_metadata.cpp:224-229); from_increments
+    infers its resolution with it (roughpy/src/streams/
+    lie_increment_stream.cpp:193-206)."""
+    _, e = math.frexp(arg)
+    return -min(5, e - 1)
+
+
 class _DI:
     """Clopen dyadic interval [k/2^n, (k+1)/2^n) (dyadic_interval.cpp:47-183)."""
 
@@ -574,7 +583,7 @@ class IncrementStream:
             for idx in indices:
                 if idx - 0.0 < min_diff:
                     min_diff = idx - 0.0
-            resolution = param_to_resolution(min_diff) + 1
+            resolution = py_param_to_resolution(min_diff) + 1
         self.resolution = int(resolution)
         self.params: List[float] = []
         self.rows: List[np.ndarray] = []

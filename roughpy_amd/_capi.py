@@ -69,19 +69,22 @@ SIGNATURES = {
     "rpy_l2t_matrix": (ctypes.c_int, [_LB, _p_i64, _p_i64, _p_i64, _p_dbl]),
     "rpy_t2l_matrix": (ctypes.c_int, [_LB, _p_i64, _p_i64, _p_i64, _p_dbl]),
     "rpy_dense_ft_fma": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _DB, _DB, _DB,
-                                        _i64, _i64, _i64, _i64, _vp]),
+                                        _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
     "rpy_dense_ft_inplace_mul": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _DB, _DB,
-                                                _i64, _i64, _i64, _vp]),
-    "rpy_dense_ft_antipode": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
-                                             _vp]),
-    "rpy_dense_ft_adjoint_left_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64,
-                                                     _i64, _i64, _i64, _vp]),
-    "rpy_dense_ft_exp": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64, _vp]),
-    "rpy_dense_ft_fmexp": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
-                                          _i64, _vp]),
-    "rpy_dense_ft_log": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64, _vp]),
+                                                _i64, _i64, _i64, _i64, _i64, _vp]),
+    "rpy_dense_ft_antipode": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _DB, _DB, _i64, _i64,
+                                             _i64, _i64, _i64, _vp]),
+    "rpy_dense_ft_adjoint_left_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _DB, _DB,
+                                                     _DB, _i64, _i64, _i64, _i64, _i64, _i64,
+                                                     _i64, _vp]),
+    "rpy_dense_ft_exp": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _DB, _DB, _i64, _i64, _i64,
+                                        _i64, _i64, _vp]),
+    "rpy_dense_ft_fmexp": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _DB, _DB, _DB, _i64,
+                                          _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
+    "rpy_dense_ft_log": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _DB, _DB, _i64, _i64, _i64,
+                                        _i64, _i64, _vp]),
     "rpy_dense_sparse_map": (ctypes.c_int, [_vp, _vp, _SM, ctypes.c_int, _dbl, _i64, _i64,
-                                            _i64, _vp]),
+                                            _i64, _i64, _i64, _vp]),
     "rpy_stream_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
                                                   ctypes.c_int, _vp]),
     "rpy_stream_log_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
@@ -109,11 +112,11 @@ SIGNATURES = {
     "rpy_debug_vjp_host_emulation": (ctypes.c_int, [_p_dbl, _p_dbl, _i64, _i32, _i32, _i32,
                                                     _p_dbl, _p_dbl]),
     "rpy_dense_st_fma": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
-                                        _i64, _vp]),
+                                        _i64, _i64, _i64, _i64, _vp]),
     "rpy_dense_st_adj_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64, _i64, _i64,
-                                            _i64, _vp]),
+                                            _i64, _i64, _i64, _i64, _vp]),
     "rpy_dense_ft_adjoint_right_mul": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _TB, _i64,
-                                                      _i64, _i64, _i64, _vp]),
+                                                      _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
 }
 
 

@@ -85,7 +85,8 @@ def _make_mul_fn():
             B = _capi.DegreeBoundsC
             _capi.check(_capi.lib().rpy_dense_ft_fma(
                 out.data_ptr(), a.data_ptr(), b.data_ptr(), _code(a), basis.c_struct(),
-                B(0, -1), B(0, -1), B(0, -1), n, size, size, size, _device.stream_handle()))
+                B(0, -1), B(0, -1), B(0, -1), n, size, size, size, 1, 1, 1,
+                _device.stream_handle()))
             ctx.save_for_backward(a, b)
             ctx.basis = basis
             return out
@@ -102,12 +103,14 @@ def _make_mul_fn():
                 ga = torch.zeros_like(a)
                 _capi.check(L.rpy_dense_ft_adjoint_right_mul(
                     ga.data_ptr(), b.data_ptr(), g.data_ptr(), _code(a), basis.c_struct(), n,
-                    size, size, size, _device.stream_handle()))
+                    size, size, size, 1, 1, 1, _device.stream_handle()))
             if ctx.needs_input_grad[1]:
                 gb = torch.zeros_like(b)
+                B = _capi.DegreeBoundsC
                 _capi.check(L.rpy_dense_ft_adjoint_left_mul(
-                    gb.data_ptr(), a.data_ptr(), g.data_ptr(), _code(b), basis.c_struct(), n,
-                    size, size, size, _device.stream_handle()))
+                    gb.data_ptr(), a.data_ptr(), g.data_ptr(), _code(b), basis.c_struct(),
+                    B(0, -1), B(0, -1), B(0, -1), n, size, size, size, 1, 1, 1,
+                    _device.stream_handle()))
             return ga, gb, None
 
     return _FtMul
@@ -120,7 +123,7 @@ def _make_map_fn():
         out = torch.zeros((x.shape[0], mat.rows), dtype=x.dtype, device=x.device)
         _capi.check(_capi.lib().rpy_dense_sparse_map(
             out.data_ptr(), x.data_ptr(), mat.device_struct(), _code(x), 1.0, x.shape[0],
-            mat.rows, x.shape[1], _device.stream_handle()))
+            mat.rows, x.shape[1], 1, 1, _device.stream_handle()))
         return out
 
     class _Map(torch.autograd.Function):

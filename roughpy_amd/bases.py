@@ -16,6 +16,19 @@ from . import _capi
 # (l2t or t2l, width, depth, Hall data) -> float64 SparseMatrix
 _MAP_CACHE = {}
 
+# 64-bit FNV-1a parameters (the published FNV offset basis and prime)
+_FNV_OFFSET_64 = 0xCBF29CE484222325
+_FNV_PRIME_64 = 0x00000100000001B3
+_U64 = (1 << 64) - 1
+
+
+def fnv1a64(data: bytes, state: int = _FNV_OFFSET_64) -> int:
+    """64-bit FNV-1a of `data` continuing from `state`: per byte, xor it into
+    the state, then multiply by the FNV prime modulo 2^64."""
+    for byte in data:
+        state = ((state ^ byte) * _FNV_PRIME_64) & _U64
+    return state
+
 
 class TensorBasis:
     """Dense tensor basis: level k occupies [degree_begin[k], degree_begin[k+1])."""
@@ -55,17 +68,35 @@ class TensorBasis:
         return int(self._degree_begin[self._depth + 1])
 
     def truncate(self, new_depth: int) -> "TensorBasis":
-        return TensorBasis(self._width, min(self._depth, int(new_depth)))
+        """The basis of the same width truncated at new_depth, which must be
+        in [0, depth) (roughpy/compute/_src/tensor_basis.c:173-203)."""
+        new_depth = int(new_depth)
+        if new_depth < 0:
+            raise ValueError("new_depth must be non-negative")
+        if new_depth >= self._depth:
+            raise ValueError("new_depth must be less than depth")
+        return TensorBasis(self._width, new_depth)
 
     def __eq__(self, other):
-        return (isinstance(other, TensorBasis) and other._width == self._width
-                and other._depth == self._depth)
+        if not isinstance(other, TensorBasis):
+            return NotImplemented
+        return (other._width == self._width and other._depth == self._depth
+                and np.array_equal(other._degree_begin, self._degree_begin))
 
     def __hash__(self):
-        return hash(("TensorBasis", self._width, self._depth))
+        """64-bit FNV-1a over b"TensorBasis", the width and depth (int32) and
+        the depth + 2 degree_begin entries (pointer-sized integers), as
+        tensor_basis_hash computes it (roughpy/compute/_src/tensor_basis.c:
+        144-171); -1 is reported as -2."""
+        payload = (b"TensorBasis" + np.int32(self._width).tobytes()
+                   + np.int32(self._depth).tobytes()
+                   + np.ascontiguousarray(self._degree_begin, dtype=np.intp).tobytes())
+        state = fnv1a64(payload)
+        h = state - (1 << 64) if state >= (1 << 63) else state
+        return -2 if h == -1 else h
 
     def __repr__(self):
-        return f"TensorBasis(width={self._width}, depth={self._depth})"
+        return f"TensorBasis({self._width}, {self._depth})"
 
     def c_struct(self) -> _capi.TensorBasisC:
         db64 = np.ascontiguousarray(self._degree_begin, dtype=np.int64)

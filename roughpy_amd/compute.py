@@ -149,6 +149,8 @@ def _padded(a, basis_size: int, dtype, name: str):
 def ft_fma(a: FreeTensor, b: FreeTensor, c: FreeTensor, **kwargs):
     """Free tensor fused multiply-add: a = b * c + a (in place on a.data).
 
+    As roughpy/compute/__init__.py:145 calls it: the product is taken in
+    b's basis and truncated at c's depth (out_depth = c.basis.depth).
     ``compensated=True`` with float32 data accumulates every output
     coefficient in double and rounds it to float once."""
     _check_basis_compat(a.basis, b.basis, c.basis)
@@ -157,31 +159,44 @@ def ft_fma(a: FreeTensor, b: FreeTensor, c: FreeTensor, **kwargs):
     if kwargs.get("compensated") and dt == np.float32:
         code = _capi.RPY_F32_COMPENSATED
     _check_leading(a.data, b.data, c.data)
-    basis = a.basis
+    basis = b.basis
     size = basis.size()
     out = _Operand(a.data, dt, size, "out")
     lhs = _padded(b.data, size, dt, "lhs")
     rhs = _padded(c.data, size, dt, "rhs")
     L = _capi.lib()
     _capi.check(L.rpy_dense_ft_fma(
-        out.ptr, lhs.ptr, rhs.ptr, code, basis.c_struct(), _bounds(0, -1),
-        _bounds(0, min(b.basis.depth, basis.depth)), _bounds(0, min(c.basis.depth, basis.depth)),
-        out.batch, out.stride, lhs.stride, rhs.stride, _device.stream_handle()))
+        out.ptr, lhs.ptr, rhs.ptr, code, basis.c_struct(), _bounds(0, c.basis.depth),
+        _bounds(), _bounds(), out.batch, out.stride, lhs.stride, rhs.stride, 1, 1, 1,
+        _device.stream_handle()))
     out.finish()
 
 
 def ft_mul(a: FreeTensor, b: FreeTensor, **kwargs) -> FreeTensor:
-    """Free tensor product a * b (new tensor in a's basis)."""
+    """Free tensor product a * b in a's basis, with b read up to its own depth
+    (rhs_depth = b.basis.depth, roughpy/compute/__init__.py:161-163)."""
     _check_basis_compat(a.basis, b.basis)
     dt = _device.np_dtype(a.data)
-    _dtype_code(dt)
-    res = _new_like(_shape(a.data), dt)
-    ft_fma(FreeTensor(res, a.basis), a, b, **kwargs)
+    code = _dtype_code(dt)
+    if kwargs.get("compensated") and dt == np.float32:
+        code = _capi.RPY_F32_COMPENSATED
+    _check_leading(a.data, b.data)
+    basis = a.basis
+    size = basis.size()
+    lhs = _padded(a.data, size, dt, "lhs")
+    rhs = _padded(b.data, size, dt, "rhs")
+    res = _new_like((lhs.batch, size), dt)
+    _capi.check(_capi.lib().rpy_dense_ft_fma(
+        res.data_ptr(), lhs.ptr, rhs.ptr, code, basis.c_struct(), _bounds(), _bounds(),
+        _bounds(0, b.basis.depth), lhs.batch, size, lhs.stride, rhs.stride, 1, 1, 1,
+        _device.stream_handle()))
+    res = res.reshape(_shape(a.data)[:-1] + (size,))
     return DenseFreeTensor(_wrap_result(res, a.data), a.basis)
 
 
 def ft_inplace_mul(a: FreeTensor, b: FreeTensor, **kwargs):
-    """a = a * b in place."""
+    """a = a * b in place, truncated at b's depth (out_depth = b.basis.depth,
+    roughpy/compute/__init__.py:181)."""
     _check_basis_compat(a.basis, b.basis)
     dt = _device.np_dtype(a.data)
     code = _dtype_code(dt)
@@ -191,9 +206,8 @@ def ft_inplace_mul(a: FreeTensor, b: FreeTensor, **kwargs):
     lhs = _Operand(a.data, dt, size, "lhs")
     rhs = _padded(b.data, size, dt, "rhs")
     _capi.check(_capi.lib().rpy_dense_ft_inplace_mul(
-        lhs.ptr, rhs.ptr, code, basis.c_struct(), _bounds(0, -1),
-        _bounds(0, min(b.basis.depth, basis.depth)), lhs.batch, lhs.stride, rhs.stride,
-        _device.stream_handle()))
+        lhs.ptr, rhs.ptr, code, basis.c_struct(), _bounds(0, b.basis.depth), _bounds(),
+        lhs.batch, lhs.stride, rhs.stride, 1, 1, _device.stream_handle()))
     lhs.finish()
 
 
@@ -205,8 +219,8 @@ def antipode(a: FreeTensor, **kwargs) -> FreeTensor:
     arg = _Operand(a.data, dt, size, "arg")
     res = _new_like((arg.batch, size), dt, zero=False)  # the kernel writes every coefficient
     _capi.check(_capi.lib().rpy_dense_ft_antipode(
-        res.data_ptr(), arg.ptr, code, a.basis.c_struct(), arg.batch, size, arg.stride,
-        _device.stream_handle()))
+        res.data_ptr(), arg.ptr, code, a.basis.c_struct(), _bounds(), _bounds(), arg.batch,
+        size, arg.stride, 1, 1, _device.stream_handle()))
     res = res.reshape(_shape(a.data)[:-1] + (size,))
     return FreeTensor(_wrap_result(res, a.data), a.basis)
 
@@ -221,8 +235,8 @@ def _unary_out(x, out_basis, entry, name):
     size = out_basis.size()
     arg = _padded(x.data, size, dt, name)
     res = _new_like((arg.batch, size), dt, zero=False)  # exp and log write every coefficient
-    _capi.check(entry(res.data_ptr(), arg.ptr, code, out_basis.c_struct(), arg.batch, size,
-                      arg.stride, _device.stream_handle()))
+    _capi.check(entry(res.data_ptr(), arg.ptr, code, out_basis.c_struct(), _bounds(), _bounds(),
+                      arg.batch, size, arg.stride, 1, 1, _device.stream_handle()))
     res = res.reshape(_shape(x.data)[:-1] + (size,))
     return FreeTensor(_wrap_result(res, x.data), out_basis)
 
@@ -252,26 +266,37 @@ def ft_fmexp(multiplier: FreeTensor, exponent: FreeTensor,
     exp = _padded(exponent.data, size, dt, "exponent")
     res = _new_like((mul.batch, size), dt, zero=False)  # the kernel writes every coefficient
     _capi.check(_capi.lib().rpy_dense_ft_fmexp(
-        res.data_ptr(), mul.ptr, exp.ptr, code, out_basis.c_struct(), mul.batch, size,
-        mul.stride, exp.stride, _device.stream_handle()))
+        res.data_ptr(), mul.ptr, exp.ptr, code, out_basis.c_struct(), _bounds(), _bounds(),
+        _bounds(), mul.batch, size, mul.stride, exp.stride, 1, 1, 1, _device.stream_handle()))
     res = res.reshape(_shape(multiplier.data)[:-1] + (size,))
     return FreeTensor(_wrap_result(res, multiplier.data), out_basis)
 
 
 def ft_adjoint_left_mul(op: FreeTensor, arg: ShuffleTensor) -> ShuffleTensor:
-    """Adjoint of left multiplication by `op`, applied to the shuffle tensor `arg`."""
+    """Adjoint of left multiplication by `op`, applied to the shuffle tensor `arg`.
+
+    Taken in op's basis with out_depth = arg depth, op depth, arg depth
+    (roughpy/compute/__init__.py:333-334); the result has arg's shape."""
     _check_basis_compat(op.basis, arg.basis)
     dt = _device.np_dtype(arg.data)
     code = _dtype_code(dt)
     _check_leading(op.data, arg.data)
-    size = arg.basis.size()
-    a = _Operand(arg.data, dt, size, "arg")
+    basis = op.basis
+    size = basis.size()
+    a = _padded(arg.data, size, dt, "arg")
     o = _padded(op.data, size, dt, "op")
     res = _new_like((a.batch, size), dt)
     _capi.check(_capi.lib().rpy_dense_ft_adjoint_left_mul(
-        res.data_ptr(), o.ptr, a.ptr, code, arg.basis.c_struct(), a.batch, size, o.stride,
-        a.stride, _device.stream_handle()))
-    res = res.reshape(_shape(arg.data)[:-1] + (size,))
+        res.data_ptr(), o.ptr, a.ptr, code, basis.c_struct(), _bounds(0, arg.basis.depth),
+        _bounds(0, op.basis.depth), _bounds(0, arg.basis.depth), a.batch, size, o.stride,
+        a.stride, 1, 1, 1, _device.stream_handle()))
+    asize = arg.basis.size()
+    if size != asize:
+        full = _new_like((a.batch, asize), dt)
+        n = min(size, asize)
+        full[:, :n] = res[:, :n]
+        res = full
+    res = res.reshape(_shape(arg.data)[:-1] + (asize,))
     return ShuffleTensor(_wrap_result(res, arg.data), arg.basis)
 
 
@@ -288,7 +313,7 @@ def st_fma(a: ShuffleTensor, b: ShuffleTensor, c: ShuffleTensor) -> ShuffleTenso
     rhs = _padded(c.data, size, dt, "rhs")
     _capi.check(_capi.lib().rpy_dense_st_fma(
         out.ptr, lhs.ptr, rhs.ptr, code, a.basis.c_struct(), out.batch, out.stride, lhs.stride,
-        rhs.stride, _device.stream_handle()))
+        rhs.stride, 1, 1, 1, _device.stream_handle()))
     out.finish()
     return a
 
@@ -317,7 +342,7 @@ def st_adj_mul(op: ShuffleTensor, arg: FreeTensor) -> FreeTensor:
     res = _new_like((a.batch, size), dt)
     _capi.check(_capi.lib().rpy_dense_st_adj_mul(
         res.data_ptr(), o.ptr, a.ptr, code, arg.basis.c_struct(), a.batch, size, o.stride,
-        a.stride, _device.stream_handle()))
+        a.stride, 1, 1, 1, _device.stream_handle()))
     res = res.reshape(_shape(arg.data)[:-1] + (size,))
     return FreeTensor(_wrap_result(res, arg.data), arg.basis)
 
@@ -328,7 +353,7 @@ def _apply_map(mat: SparseMatrix, x, dt, out_cols_basis_size, scale):
     res = _new_like((a.batch, mat.rows), dt)
     _capi.check(_capi.lib().rpy_dense_sparse_map(
         res.data_ptr(), a.ptr, mat.device_struct(), code, float(scale), a.batch, mat.rows,
-        a.stride, _device.stream_handle()))
+        a.stride, 1, 1, _device.stream_handle()))
     return res.reshape(_shape(x)[:-1] + (mat.rows,))
 
 

@@ -16,6 +16,7 @@
 #include "../../include/roughpy_amd.h"
 #include "chen.hpp"
 #include "dense_ops.hpp"
+#include "strided.hpp"
 #include "rpy_common.hpp"
 
 namespace rpy_amd {
@@ -29,6 +30,17 @@ __global__ void widen_kernel(double* __restrict__ out, const float* __restrict__
          i += int64_t(gridDim.x) * blockDim.x)
         out[i] = double(in[i]);
 }
+
+// Coefficients of a tensor operand (0 for an invalid basis: the entry
+// point's own checks report it).
+int64_t basis_size(const rpy_tensor_basis* b)
+{
+    if (!b || b->width < 1 || b->depth < 0 || b->depth > kMaxDepth) return 0;
+    const int64_t n = tensor_size_checked(b->width, b->depth);
+    return n < 0 ? 0 : n;
+}
+
+size_t elem_size(rpy_dtype dt) { return dt == RPY_F64 ? 8 : 4; }
 
 uint32_t blocks_for(int64_t n)
 {
@@ -170,71 +182,133 @@ int rpy_t2l_matrix(const rpy_lie_basis* basis, int64_t* nnz, int64_t* offsets, i
 int rpy_dense_ft_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
                      const rpy_tensor_basis* basis, rpy_degree_bounds ob, rpy_degree_bounds lb,
                      rpy_degree_bounds rb, int64_t batch, int64_t os, int64_t ls, int64_t rs,
-                     void* stream)
+                     int64_t oi, int64_t li, int64_t ri, void* stream)
 {
-    return dense_ft_fma(out, lhs, rhs, dtype, basis, ob, lb, rb, batch, os, ls, rs, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)lhs, ls, li, n, false},
+                                 {(void*)rhs, rs, ri, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_fma(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, ob, lb, rb, batch,
+                            v.stride[0], v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
                              const rpy_tensor_basis* basis, rpy_degree_bounds lb,
                              rpy_degree_bounds rb, int64_t batch, int64_t ls, int64_t rs,
-                             void* stream)
+                             int64_t li, int64_t ri, void* stream)
 {
-    return dense_ft_inplace_mul(lhs, rhs, dtype, basis, lb, rb, batch, ls, rs, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[2] = {{lhs, ls, li, n, true}, {(void*)rhs, rs, ri, n, false}};
+    return with_unit_inc(ops, 2, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_inplace_mul(v.ptr[0], v.ptr[1], dtype, basis, lb, rb, batch, v.stride[0],
+                                    v.stride[1], stream);
+    });
 }
 
 int rpy_dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype,
-                          const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t as,
-                          void* stream)
+                          const rpy_tensor_basis* basis, rpy_degree_bounds ob,
+                          rpy_degree_bounds ab, int64_t batch, int64_t os, int64_t as,
+                          int64_t oi, int64_t ai, void* stream)
 {
-    return dense_ft_antipode(out, arg, dtype, basis, batch, os, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[2] = {{out, os, oi, n, true}, {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 2, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_antipode(v.ptr[0], v.ptr[1], dtype, basis, ob, ab, batch, v.stride[0],
+                                 v.stride[1], stream);
+    });
 }
 
 int rpy_dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
-                                  const rpy_tensor_basis* basis, int64_t batch, int64_t os,
-                                  int64_t ps, int64_t as, void* stream)
+                                  const rpy_tensor_basis* basis, rpy_degree_bounds ob,
+                                  rpy_degree_bounds pb, rpy_degree_bounds ab, int64_t batch,
+                                  int64_t os, int64_t ps, int64_t as, int64_t oi, int64_t pi,
+                                  int64_t ai, void* stream)
 {
-    return dense_ft_adjoint_left_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)op, ps, pi, n, false},
+                                 {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_adjoint_left_mul(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, ob, pb, ab,
+                                         batch, v.stride[0], v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_dense_st_fma(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
                      const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ls,
-                     int64_t rs, void* stream)
+                     int64_t rs, int64_t oi, int64_t li, int64_t ri, void* stream)
 {
-    return dense_st_fma(out, lhs, rhs, dtype, basis, batch, os, ls, rs, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)lhs, ls, li, n, false},
+                                 {(void*)rhs, rs, ri, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_st_fma(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, batch, v.stride[0],
+                            v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_dense_st_adj_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
                          const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ps,
-                         int64_t as, void* stream)
+                         int64_t as, int64_t oi, int64_t pi, int64_t ai, void* stream)
 {
-    return dense_st_adj_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)op, ps, pi, n, false},
+                                 {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_st_adj_mul(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, batch, v.stride[0],
+                                v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                     int64_t batch, int64_t os, int64_t as, void* stream)
+                     rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                     int64_t as, int64_t oi, int64_t ai, void* stream)
 {
-    return dense_ft_exp(out, arg, dtype, basis, batch, os, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[2] = {{out, os, oi, n, true}, {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 2, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_exp(v.ptr[0], v.ptr[1], dtype, basis, ob, ab, batch, v.stride[0],
+                            v.stride[1], stream);
+    });
 }
 
 int rpy_dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
-                       const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
-                       int64_t es, void* stream)
+                       const rpy_tensor_basis* basis, rpy_degree_bounds ob,
+                       rpy_degree_bounds mb, rpy_degree_bounds eb, int64_t batch, int64_t os,
+                       int64_t ms, int64_t es, int64_t oi, int64_t mi, int64_t ei, void* stream)
 {
-    return dense_ft_fmexp(out, mul, expo, dtype, basis, batch, os, ms, es, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)mul, ms, mi, n, false},
+                                 {(void*)expo, es, ei, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_fmexp(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, ob, mb, eb, batch,
+                              v.stride[0], v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                     int64_t batch, int64_t os, int64_t as, void* stream)
+                     rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                     int64_t as, int64_t oi, int64_t ai, void* stream)
 {
-    return dense_ft_log(out, arg, dtype, basis, batch, os, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[2] = {{out, os, oi, n, true}, {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 2, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_log(v.ptr[0], v.ptr[1], dtype, basis, ob, ab, batch, v.stride[0],
+                            v.stride[1], stream);
+    });
 }
 
 int rpy_dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* matrix,
                          rpy_dtype dtype, double scale, int64_t batch, int64_t os, int64_t as,
-                         void* stream)
+                         int64_t oi, int64_t ai, void* stream)
 {
-    return dense_sparse_map(out, arg, matrix, dtype, scale, batch, os, as, stream);
+    if (!matrix) return fail(RPY_ERR_INVALID_ARGUMENT, "null matrix");
+    const DenseOperand ops[2] = {{out, os, oi, matrix->rows, true},
+                                 {(void*)arg, as, ai, matrix->cols, false}};
+    return with_unit_inc(ops, 2, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_sparse_map(v.ptr[0], v.ptr[1], matrix, dtype, scale, batch, v.stride[0],
+                                v.stride[1], stream);
+    });
 }
 
 int rpy_stream_signature_batch(void* out, const void* increments, const int64_t* offsets,
@@ -262,13 +336,16 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const size_t tel = size_t(n_streams) * size_t(tsize);
     const size_t elt = (dtype == RPY_F64) ? 8 : 4;
-    // sig (dtype) | sig widened to f64 (if needed) | log (f64)
+    // sig (dtype) | sig widened to f64 (if needed) | log (f64), each segment
+    // starting on a 256-byte boundary
+    auto seg = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    const size_t sig_bytes = seg(tel * elt);
+    const size_t wide_bytes = dtype == RPY_F64 ? 0 : seg(tel * 8);
     char* ws = nullptr;
-    const size_t bytes = tel * elt + (dtype == RPY_F64 ? 0 : tel * 8) + tel * 8;
-    RPY_HIP_CHECK(workspace_alloc((void**)&ws, bytes, st));
+    RPY_HIP_CHECK(workspace_alloc((void**)&ws, sig_bytes + wide_bytes + seg(tel * 8), st));
     void* sig = ws;
-    double* sig64 = (dtype == RPY_F64) ? (double*)ws : (double*)(ws + tel * elt);
-    double* lg = (double*)(ws + tel * elt + (dtype == RPY_F64 ? 0 : tel * 8));
+    double* sig64 = (dtype == RPY_F64) ? (double*)ws : (double*)(ws + sig_bytes);
+    double* lg = (double*)(ws + sig_bytes + wide_bytes);
     int rc = stream_signature_batch(sig, increments, offsets, n_streams, width, depth, dtype,
                                     stream);
     rpy_tensor_basis tb{width, depth, nullptr};
@@ -285,7 +362,9 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
             if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
         }
     }
-    if (!rc && !logged) rc = dense_ft_log(lg, sig64, RPY_F64, &tb, n_streams, tsize, tsize, stream);
+    if (!rc && !logged)
+        rc = dense_ft_log(lg, sig64, RPY_F64, &tb, {0, -1}, {0, -1}, n_streams, tsize, tsize,
+                          stream);
     // the projection writes the output (no clearing pass, no read of it)
     if (!rc)
         rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize, stream,
@@ -361,9 +440,16 @@ int rpy_stream_signature_vjp(void* grad_increments, const void* increments,
 
 int rpy_dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
                                    const rpy_tensor_basis* basis, int64_t batch, int64_t os,
-                                   int64_t ps, int64_t as, void* stream)
+                                   int64_t ps, int64_t as, int64_t oi, int64_t pi, int64_t ai,
+                                   void* stream)
 {
-    return dense_ft_adjoint_right_mul(out, op, arg, dtype, basis, batch, os, ps, as, stream);
+    const int64_t n = basis_size(basis);
+    const DenseOperand ops[3] = {{out, os, oi, n, true}, {(void*)op, ps, pi, n, false},
+                                 {(void*)arg, as, ai, n, false}};
+    return with_unit_inc(ops, 3, batch, elem_size(dtype), stream, [&](const UnitIncViews& v) {
+        return dense_ft_adjoint_right_mul(v.ptr[0], v.ptr[1], v.ptr[2], dtype, basis, batch,
+                                          v.stride[0], v.stride[1], v.stride[2], stream);
+    });
 }
 
 int rpy_signature_gram(float* gram, const float* signatures, int64_t n, int64_t k,

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 
+#include "chen.hpp"
 #include "chen_core.hpp"
 #include "dense_ops.hpp"
 #include "kernel_util.hpp"
@@ -201,6 +202,25 @@ __global__ __launch_bounds__(kBlock) void ft_inplace_mul_kernel(
     }
 }
 
+// dst[e, i] = src[e, i] for keep_begin <= i < keep_end, else 0, over n
+// coefficients per element (dst contiguous, src with batch stride ss): an
+// operand whose degree bounds cut inside the levels a call reads.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void truncate_copy_kernel(T* __restrict__ dst,
+                                                              const T* __restrict__ src,
+                                                              int64_t batch, int64_t n,
+                                                              int64_t ss, int64_t keep_begin,
+                                                              int64_t keep_end)
+{
+    const int64_t total = batch * n;
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total;
+         t += int64_t(gridDim.x) * kBlock) {
+        const int64_t e = t / n;
+        const int64_t i = t - e * n;
+        dst[t] = (i >= keep_begin && i < keep_end) ? src[e * ss + i] : T(0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Horner forms of exp / fmexp / log on one element held by one workgroup.
 // `a` is the working tensor (LDS or the output buffer), `x` the argument.
@@ -287,19 +307,22 @@ __global__ __launch_bounds__(kBlock) void horner_kernel(
 // ---------------------------------------------------------------------------
 // antipode: out[rev(w)] = (-1)^|w| arg[w]
 // ---------------------------------------------------------------------------
+// Writes the levels lo..hi of each element (the whole tensor when lo = 0
+// and hi = depth); other levels of out are left as they are.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void antipode_kernel(T* __restrict__ out,
                                                          const T* __restrict__ arg, DevBasis b,
                                                          Pow pw, int64_t batch, int64_t os,
-                                                         int64_t as)
+                                                         int64_t as, int lo, int hi)
 {
-    const int64_t n = b.size();
+    const int64_t g0 = b.db[lo];
+    const int64_t n = b.db[hi + 1] - g0;
     const int64_t total = batch * n;
     const uint32_t w = uint32_t(b.width);
     for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total;
          t += int64_t(gridDim.x) * kBlock) {
         const int64_t e = t / n;
-        const int64_t g = t - e * n;
+        const int64_t g = g0 + (t - e * n);
         int d = 0;
         while (g >= b.db[d + 1]) ++d;
         uint32_t i = uint32_t(g - b.db[d]);
@@ -317,27 +340,38 @@ __global__ __launch_bounds__(kBlock) void antipode_kernel(T* __restrict__ out,
 // ---------------------------------------------------------------------------
 // out_k[i] += sum_{|u| = p} op_p[u] arg_{k+p}[u i]
 // ---------------------------------------------------------------------------
+// Degree ranges of the adjoint: output degrees olo..ohi, op degrees
+// plo..phi, and argument degrees amin..amax, the latter computed as
+// free_tensor_adjoint_left_mul.hpp:39-40 does from the three operands' bounds
+// (amax = min(arg max - op min, out max), amin = max(arg min - op max,
+// out min)).
+struct AdjRange {
+    int olo, ohi, plo, phi, amin, amax;
+};
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void adj_lmul_kernel(T* __restrict__ out,
                                                          const T* __restrict__ op,
                                                          const T* __restrict__ arg, DevBasis b,
                                                          int64_t batch, int64_t os, int64_t ps,
-                                                         int64_t as)
+                                                         int64_t as, AdjRange r)
 {
-    const int D = b.depth;
-    const int64_t n = b.size();
+    const int64_t g0 = b.db[r.olo];
+    const int64_t g1 = b.db[r.ohi + 1];
     for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
         const T* p = op + e * ps;
         const T* a = arg + e * as;
         T* o = out + e * os;
-        for (int64_t g = threadIdx.x; g < n; g += kBlock) {
+        for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
             int od = 0;
             while (g >= b.db[od + 1]) ++od;
             const int64_t i = g - b.db[od];
             const int64_t osz = b.level_size(od);
             T acc = o[g];
             // argument degree descending == op degree descending
-            for (int pd = D - od; pd >= 0; --pd) {
+            const int pd_hi = min(r.phi, r.amax - od);
+            const int pd_lo = max(r.plo, r.amin - od);
+            for (int pd = pd_hi; pd >= pd_lo; --pd) {
                 const int ad = od + pd;
                 const int64_t psz = b.level_size(pd);
                 for (int64_t u = 0; u < psz; ++u)
@@ -662,12 +696,16 @@ __global__ __launch_bounds__(kBlock) void adj_lmul_lds_kernel(T* __restrict__ ou
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// max_degree -1 or above the depth means the depth, as update_algebra_params
+// (roughpy/compute/_src/call_config.cpp:11-31) resolves it; a min_degree
+// above the resolved max_degree (or a negative one) is an error.
 int check_bounds(const DevBasis& b, rpy_degree_bounds in, Bounds& out, const char* name)
 {
     out.lo = in.min_degree;
-    out.hi = in.max_degree < 0 ? b.depth : in.max_degree;
-    if (out.lo < 0 || out.hi > b.depth || out.lo > out.hi)
-        return fail(RPY_ERR_INVALID_ARGUMENT, "invalid degree bounds for %s: [%d, %d] with depth %d",
+    out.hi = (in.max_degree == -1 || in.max_degree > b.depth) ? b.depth : in.max_degree;
+    if (out.lo < 0 || out.hi < 0 || out.lo > out.hi)
+        return fail(RPY_ERR_INVALID_ARGUMENT,
+                    "invalid degree bounds for %s: min_degree %d, max_degree %d (depth %d)",
                     name, in.min_degree, in.max_degree, b.depth);
     return RPY_OK;
 }
@@ -1220,18 +1258,29 @@ int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
 }
 
 int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                      int64_t batch, int64_t os, int64_t as, void* stream)
+                      rpy_degree_bounds obi, rpy_degree_bounds abi, int64_t batch, int64_t os,
+                      int64_t as, void* stream)
 {
     DevBasis b;
     int rc;
     if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    Bounds ob, ab;
+    if ((rc = check_bounds(b, obi, ob, "out")) || (rc = check_bounds(b, abi, ab, "arg")))
+        return rc;
     if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     const Pow pw = make_pow(b.width, 1);
+    // Levels written: max(mins) .. min(maxes); level 1 whenever that range
+    // starts at or below 1 (free_tensor_antipode.hpp:60-80 writes it
+    // unconditionally then).
+    const int lo = std::max(ob.lo, ab.lo);
+    int hi = std::min(ob.hi, ab.hi);
+    if (lo <= 1 && b.depth >= 1) hi = std::max(hi, 1);
+    if (lo > hi) return RPY_OK;
     const size_t es = dtype == RPY_F64 ? 8 : 4;
     const size_t lds = size_t((b.size() + 31) / 32 * 32) * es;
-    if (lds <= kLdsBudget) {
+    if (lo == 0 && hi == b.depth && lds <= kLdsBudget) {
         dim3 grid(grid_for(batch)), block(kBlock);
         if (dtype == RPY_F64)
             hipLaunchKernelGGL(antipode_lds_kernel<double>, grid, block, lds, as_stream(stream),
@@ -1242,13 +1291,13 @@ int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_ten
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
     }
-    dim3 grid(blocks_elementwise(batch * b.size())), block(kBlock);
+    dim3 grid(blocks_elementwise(batch * (b.db[hi + 1] - b.db[lo]))), block(kBlock);
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(antipode_kernel<double>, grid, block, 0, as_stream(stream),
-                           (double*)out, (const double*)arg, b, pw, batch, os, as);
+                           (double*)out, (const double*)arg, b, pw, batch, os, as, lo, hi);
     else
         hipLaunchKernelGGL(antipode_kernel<float>, grid, block, 0, as_stream(stream),
-                           (float*)out, (const float*)arg, b, pw, batch, os, as);
+                           (float*)out, (const float*)arg, b, pw, batch, os, as, lo, hi);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }
@@ -1486,16 +1535,23 @@ AdjlFixed<T> find_adjl_fixed(int w, int d)
 }
 
 int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
-                              const rpy_tensor_basis* basis, int64_t batch, int64_t os,
-                              int64_t ps, int64_t as, void* stream)
+                              const rpy_tensor_basis* basis, rpy_degree_bounds obi,
+                              rpy_degree_bounds pbi, rpy_degree_bounds abi, int64_t batch,
+                              int64_t os, int64_t ps, int64_t as, void* stream)
 {
     DevBasis b;
     int rc;
     if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    Bounds ob, pb, ab;
+    if ((rc = check_bounds(b, obi, ob, "out")) || (rc = check_bounds(b, pbi, pb, "op")) ||
+        (rc = check_bounds(b, abi, ab, "arg")))
+        return rc;
     if ((rc = check_batch(batch, {os, ps, as}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !op || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
-    {
+    const bool full = ob.lo == 0 && pb.lo == 0 && ab.lo == 0 && ob.hi == b.depth &&
+                      pb.hi == b.depth && ab.hi == b.depth;
+    if (full) {
         const auto f64 = find_adjl_fixed<double>(b.width, b.depth);
         const auto f32 = find_adjl_fixed<float>(b.width, b.depth);
         const bool done = dtype == RPY_F64
@@ -1509,7 +1565,7 @@ int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dt
     dim3 grid(grid_for(batch)), block(kBlock);
     const size_t es = dtype == RPY_F64 ? 8 : 4;
     const size_t lds = 2 * size_t((b.size() + 3) / 4 * 4) * es;
-    if (lds <= kLdsBudget && b.depth <= kMaxDepth) {
+    if (full && lds <= kLdsBudget && b.depth <= kMaxDepth) {
         const Pow pw = make_pow(b.width, b.depth);
         const AdjLPlan plan = make_adjl_plan(b);
         if (dtype == RPY_F64)
@@ -1523,64 +1579,144 @@ int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dt
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
     }
+    const AdjRange r{ob.lo, ob.hi, pb.lo, pb.hi, std::max(ab.lo - pb.hi, ob.lo),
+                     std::min(ab.hi - pb.lo, ob.hi)};
+    if (r.amin > r.amax) return RPY_OK;
     if (dtype == RPY_F64)
         hipLaunchKernelGGL(adj_lmul_kernel<double>, grid, block, 0, as_stream(stream),
                            (double*)out, (const double*)op, (const double*)arg, b, batch, os, ps,
-                           as);
+                           as, r);
     else
         hipLaunchKernelGGL(adj_lmul_kernel<float>, grid, block, 0, as_stream(stream),
                            (float*)out, (const float*)op, (const float*)arg, b, batch, os, ps,
-                           as);
+                           as, r);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }
 
+// exp / fmexp / log of the first db[m + 1] coefficients, m = out's max
+// degree: the computation of basis (width, m), whose dense layout is that
+// prefix; out's coefficients above degree m are left as they are.  An
+// argument (or multiplier) whose degree bounds cut inside the levels the
+// series reads (1..m for the exponent and the log's argument, 0..m for the
+// multiplier) is read through a copy with the coefficients outside its
+// bounds set to zero.
 static int horner_entry(Horner h, void* out, const void* mul, const void* arg, rpy_dtype dtype,
-                        const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
-                        int64_t as, void* stream)
+                        const rpy_tensor_basis* basis, rpy_degree_bounds obi,
+                        rpy_degree_bounds mbi, rpy_degree_bounds abi, int64_t batch, int64_t os,
+                        int64_t ms, int64_t as, void* stream)
 {
     DevBasis b;
     int rc;
     if ((rc = make_basis(basis, b)) || (rc = plain_dtype(dtype))) return rc;
+    Bounds ob, mb{0, b.depth}, ab;
+    if ((rc = check_bounds(b, obi, ob, "out")) || (rc = check_bounds(b, abi, ab, "arg")) ||
+        (h == Horner::FmExp && (rc = check_bounds(b, mbi, mb, "multiplier"))))
+        return rc;
+    if (ob.lo != 0)
+        return fail(RPY_ERR_INVALID_ARGUMENT,
+                    "the output of exp, fmexp and log starts at degree 0 (min_degree %d)", ob.lo);
     if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
     if (h == Horner::FmExp && (rc = check_batch(batch, {ms}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !arg || (h == Horner::FmExp && !mul))
         return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     hipStream_t st = as_stream(stream);
-    if (dtype == RPY_F64) {
-        switch (h) {
-            case Horner::Exp: return launch_horner<double, Horner::Exp>(out, mul, arg, b, batch, os, ms, as, st);
-            case Horner::FmExp: return launch_horner<double, Horner::FmExp>(out, mul, arg, b, batch, os, ms, as, st);
-            case Horner::Log: return launch_horner<double, Horner::Log>(out, mul, arg, b, batch, os, ms, as, st);
-        }
-    } else {
-        switch (h) {
-            case Horner::Exp: return launch_horner<float, Horner::Exp>(out, mul, arg, b, batch, os, ms, as, st);
-            case Horner::FmExp: return launch_horner<float, Horner::FmExp>(out, mul, arg, b, batch, os, ms, as, st);
-            case Horner::Log: return launch_horner<float, Horner::Log>(out, mul, arg, b, batch, os, ms, as, st);
+    const int m = ob.hi;
+    DevBasis bm;
+    {
+        rpy_tensor_basis tb{b.width, m, nullptr};
+        if ((rc = make_basis(&tb, bm))) return rc;
+    }
+    const int64_t nm = bm.size();
+    const bool cut_arg = ab.lo > 1 || ab.hi < m;
+    const bool cut_mul = h == Horner::FmExp && (mb.lo > 0 || mb.hi < m);
+    const size_t es = dtype == RPY_F64 ? 8 : 4;
+    char* ws = nullptr;
+    const size_t seg = (size_t(batch) * size_t(nm) * es + 255) & ~size_t(255);
+    if (cut_arg || cut_mul)
+        RPY_HIP_CHECK(workspace_alloc((void**)&ws, seg * ((cut_arg ? 1 : 0) + (cut_mul ? 1 : 0)),
+                                      st));
+    const void* argp = arg;
+    const void* mulp = mul;
+    int64_t as2 = as, ms2 = ms;
+    auto cut = [&](const void* src, int64_t ss, Bounds bd, void* dst) {
+        const int64_t kb = bm.db[std::min(bd.lo, m + 1)];
+        const int64_t ke = bm.db[std::min(bd.hi, m) + 1];
+        dim3 grid(blocks_elementwise(batch * nm)), block(kBlock);
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL(truncate_copy_kernel<double>, grid, block, 0, st, (double*)dst,
+                               (const double*)src, batch, nm, ss, kb, ke);
+        else
+            hipLaunchKernelGGL(truncate_copy_kernel<float>, grid, block, 0, st, (float*)dst,
+                               (const float*)src, batch, nm, ss, kb, ke);
+    };
+    if (cut_arg) {
+        cut(arg, as, ab, ws);
+        argp = ws;
+        as2 = nm;
+    }
+    if (cut_mul) {
+        void* dst = ws + (cut_arg ? seg : 0);
+        cut(mul, ms, mb, dst);
+        mulp = dst;
+        ms2 = nm;
+    }
+    rc = RPY_OK;
+    if ((cut_arg || cut_mul) && hipGetLastError() != hipSuccess)
+        rc = fail(RPY_ERR_HIP, "truncating copy launch failed");
+    if (!rc) {
+        if (dtype == RPY_F64) {
+            switch (h) {
+                case Horner::Exp: rc = launch_horner<double, Horner::Exp>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+                case Horner::FmExp: rc = launch_horner<double, Horner::FmExp>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+                case Horner::Log: rc = launch_horner<double, Horner::Log>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+            }
+        } else {
+            switch (h) {
+                case Horner::Exp: rc = launch_horner<float, Horner::Exp>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+                case Horner::FmExp: rc = launch_horner<float, Horner::FmExp>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+                case Horner::Log: rc = launch_horner<float, Horner::Log>(out, mulp, argp, bm, batch, os, ms2, as2, st); break;
+            }
         }
     }
-    return fail(RPY_ERR_INVALID_ARGUMENT, "bad Horner kind");
+    if (ws) {
+        hipError_t ferr = hipFreeAsync(ws, st);
+        if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    }
+    return rc;
 }
 
 int dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                 int64_t batch, int64_t os, int64_t as, void* stream)
+                 rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                 int64_t as, void* stream)
 {
-    return horner_entry(Horner::Exp, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+    return horner_entry(Horner::Exp, out, nullptr, arg, dtype, basis, ob, {0, -1}, ab, batch, os,
+                        0, as, stream);
 }
 
 int dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
-                   const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
-                   int64_t es, void* stream)
+                   const rpy_tensor_basis* basis, rpy_degree_bounds ob, rpy_degree_bounds mb,
+                   rpy_degree_bounds eb, int64_t batch, int64_t os, int64_t ms, int64_t es,
+                   void* stream)
 {
-    return horner_entry(Horner::FmExp, out, mul, expo, dtype, basis, batch, os, ms, es, stream);
+    return horner_entry(Horner::FmExp, out, mul, expo, dtype, basis, ob, mb, eb, batch, os, ms,
+                        es, stream);
 }
 
 int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                 int64_t batch, int64_t os, int64_t as, void* stream)
+                 rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                 int64_t as, void* stream)
 {
-    return horner_entry(Horner::Log, out, nullptr, arg, dtype, basis, batch, os, 0, as, stream);
+    return horner_entry(Horner::Log, out, nullptr, arg, dtype, basis, ob, {0, -1}, ab, batch, os,
+                        0, as, stream);
+}
+
+bool has_fixed_mul(int32_t width, int32_t depth, rpy_dtype dtype)
+{
+    if (dtype == RPY_F64) return find_mul_fixed<double>(width, depth) != nullptr;
+    if (dtype == RPY_F32) return find_mul_fixed<float>(width, depth) != nullptr;
+    return false;
 }
 
 int dense_ft_mul_new(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,
@@ -1666,8 +1802,9 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
             if (lds > 64 * 1024) {
                 static std::atomic<bool> reserved{false};
                 if (!reserved.load(std::memory_order_acquire)) {
-                    hipFuncSetAttribute((const void*)kern,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+                    (void)hipFuncSetAttribute((const void*)kern,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              128 * 1024);
                     reserved.store(true, std::memory_order_release);
                 }
             }

@@ -16,17 +16,24 @@ int dense_ft_inplace_mul(void* lhs, const void* rhs, rpy_dtype dtype,
                          rpy_degree_bounds rb, int64_t batch, int64_t ls, int64_t rs,
                          void* stream);
 int dense_ft_antipode(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                      int64_t batch, int64_t os, int64_t as, void* stream);
+                      rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                      int64_t as, void* stream);
 int dense_ft_adjoint_left_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
-                              const rpy_tensor_basis* basis, int64_t batch, int64_t os,
-                              int64_t ps, int64_t as, void* stream);
+                              const rpy_tensor_basis* basis, rpy_degree_bounds ob,
+                              rpy_degree_bounds pb, rpy_degree_bounds ab, int64_t batch,
+                              int64_t os, int64_t ps, int64_t as, void* stream);
 int dense_ft_exp(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                 int64_t batch, int64_t os, int64_t as, void* stream);
+                 rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                 int64_t as, void* stream);
 int dense_ft_fmexp(void* out, const void* mul, const void* expo, rpy_dtype dtype,
-                   const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t ms,
-                   int64_t es, void* stream);
+                   const rpy_tensor_basis* basis, rpy_degree_bounds ob, rpy_degree_bounds mb,
+                   rpy_degree_bounds eb, int64_t batch, int64_t os, int64_t ms, int64_t es,
+                   void* stream);
 int dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tensor_basis* basis,
-                 int64_t batch, int64_t os, int64_t as, void* stream);
+                 rpy_degree_bounds ob, rpy_degree_bounds ab, int64_t batch, int64_t os,
+                 int64_t as, void* stream);
+// Whether dense_ft_mul_new has a fixed-shape product for (width, depth, dtype).
+bool has_fixed_mul(int32_t width, int32_t depth, rpy_dtype dtype);
 // out = lhs (x) rhs at full depth into a buffer of its own (out is written,
 // not read); RPY_ERR_UNSUPPORTED for shapes without a fixed-shape kernel.
 int dense_ft_mul_new(void* out, const void* lhs, const void* rhs, rpy_dtype dtype,

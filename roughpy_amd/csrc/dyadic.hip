@@ -658,10 +658,11 @@ int dyadic_cache_build(void* cache, const void* increments, const double* params
         char* outp = base + size_t(node_offset(j, n_streams, 0, 0) * tsize) * es;
         const char* lhs = base + size_t(node_offset(j + 1, n_streams, 0, 0) * tsize) * es;
         // fixed shapes write the parents directly; others clear them and add
-        rc = dense_ft_mul_new(outp, lhs, lhs + size_t(tsize) * es, dtype, &tb, batch, tsize,
-                              2 * tsize, 2 * tsize, stream);
-        if (rc != RPY_ERR_UNSUPPORTED) continue;
-        rc = RPY_OK;
+        if (has_fixed_mul(width, depth, dtype)) {
+            rc = dense_ft_mul_new(outp, lhs, lhs + size_t(tsize) * es, dtype, &tb, batch, tsize,
+                                  2 * tsize, 2 * tsize, stream);
+            continue;
+        }
         if (dtype == RPY_F64)
             hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(blocks_for(batch * tsize)), dim3(kBlock),
                                0, st, (double*)outp, batch * tsize);

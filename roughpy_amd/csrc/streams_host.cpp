@@ -80,7 +80,9 @@ int dyadic_dissection(double inf, double sup, int32_t tol, std::vector<DI>& out)
     out.clear();
     if (!(std::isfinite(inf) && std::isfinite(sup)))
         return fail(RPY_ERR_INVALID_ARGUMENT, "interval end points must be finite");
-    if (tol < 0 || tol > 60) return fail(RPY_ERR_INVALID_ARGUMENT, "resolution out of range");
+    // negative resolutions are pieces longer than 1 (a stream whose
+    // parameters are 2 or more apart infers one)
+    if (tol < -60 || tol > 60) return fail(RPY_ERR_INVALID_ARGUMENT, "resolution out of range");
     if (std::fabs(std::ldexp(inf, tol)) > 9.0e15 || std::fabs(std::ldexp(sup, tol)) > 9.0e15)
         return fail(RPY_ERR_INVALID_ARGUMENT, "interval too large for the resolution");
     DI begin = from_param(inf, tol);

@@ -27,7 +27,7 @@ from .compute import DenseFreeTensor, DenseLie
 __all__ = [
     "RealInterval", "Partition", "LieIncrementStream", "PiecewiseAbelianStream",
     "DyadicSignatureCache", "signature_batch", "log_signature_batch",
-    "param_to_resolution", "to_dyadic_intervals",
+    "param_to_resolution", "py_param_to_resolution", "to_dyadic_intervals",
 ]
 
 
@@ -236,6 +236,16 @@ def _cbh_rows(lies, tb: TensorBasis, lb: LieBasis):
     return rc.tensor_to_lie(rc.FreeTensor(logc, tb), lb).data
 
 
+def py_param_to_resolution(accuracy: float) -> int:
+    """-min(5, e - 1) for accuracy = m 2^e with 0.5 <= |m| < 1: the Python
+    bindings' conversion of an accuracy to a resolution, used for the
+    resolution from_increments infers and for float resolution arguments
+    (roughpy/src/args/kwargs_to_path_metadata.cpp:224-229; the streams
+    library's own param_to_resolution caps at 0 instead of 5)."""
+    _, e = math.frexp(float(accuracy))
+    return -min(5, e - 1)
+
+
 def _resolution_arg(res) -> Optional[int]:
     if res is None:
         return None
@@ -244,8 +254,42 @@ def _resolution_arg(res) -> Optional[int]:
     if isinstance(res, (int, np.integer)):
         return int(res)
     if isinstance(res, (float, np.floating)):
-        return param_to_resolution(float(res))
+        return py_param_to_resolution(float(res))
     raise TypeError("resolution should be either float or int")
+
+
+_VTYPES = ("dense", "sparse")
+
+
+def _check_vtype(vtype):
+    """vtype as the reference takes it (algebra::VectorType, Dense or Sparse):
+    a name, or anything whose name attribute is one."""
+    if vtype is None:
+        return
+    name = getattr(vtype, "name", vtype)
+    if not isinstance(name, str) or name.lower() not in _VTYPES:
+        raise TypeError(f"vtype must be one of {_VTYPES}, got {vtype!r}")
+
+
+def _schema_increment_channels(schema, channel_types):
+    """Number of channels a schema or channel_types argument describes when
+    every channel is an increment (the only kind a LieIncrementStream of
+    this path carries), None when neither is given or the channels are
+    homogeneous increments of the data's width."""
+    spec = schema if schema is not None else channel_types
+    if spec is None:
+        return None
+    if isinstance(spec, str):
+        if spec.lower() == "increment":
+            return None
+        raise NotImplementedError(f"channel type {spec!r} is not supported")
+    items = list(spec.values()) if isinstance(spec, dict) else list(spec)
+    for it in items:
+        name = getattr(it, "name", it)
+        if not isinstance(name, str) or name.lower() != "increment":
+            raise NotImplementedError(
+                f"only increment channels are supported, got {it!r}")
+    return len(items)
 
 
 def _dtype_code(dt) -> int:
@@ -255,6 +299,33 @@ def _dtype_code(dt) -> int:
     if dt == np.float32:
         return _capi.RPY_F32
     raise TypeError(f"unsupported dtype {dt}")
+
+
+def _rows_of(inc, width: int):
+    """(rows, width) view of the increments: a 2-d array must have `width`
+    columns; a 1-d one is read as consecutive rows."""
+    if inc.ndim == 2:
+        if int(inc.shape[1]) != int(width):
+            raise ValueError(f"increments have {int(inc.shape[1])} columns, width is {width}")
+        return inc
+    if inc.ndim == 1 and int(inc.shape[0]) % int(width) == 0:
+        return inc.reshape(-1, int(width))
+    raise ValueError(f"increments of shape {tuple(inc.shape)} are not rows of width {width}")
+
+
+def _check_host_offsets(offsets, rows: int):
+    """CSR offsets given on the host are checked before any launch: they
+    start at or above 0, never decrease and end at or below the row count.
+    Offsets already on the device are taken as they are (the caller
+    guarantees the same)."""
+    if _device.is_device_array(offsets):
+        return
+    off = np.asarray(offsets, dtype=np.int64).reshape(-1)
+    if off.shape[0] < 1:
+        raise ValueError("offsets must hold at least one entry")
+    if off[0] < 0 or np.any(np.diff(off) < 0) or off[-1] > rows:
+        raise ValueError("offsets must be non-decreasing, start at 0 or above and end at "
+                         f"or below the number of increment rows ({rows})")
 
 
 def signature_batch(increments, offsets, width: int, depth: int, dtype=None,
@@ -273,9 +344,8 @@ def signature_batch(increments, offsets, width: int, depth: int, dtype=None,
         if dt != np.float32:
             raise TypeError("compensated signatures take float32 data")
         code = _capi.RPY_F32_COMPENSATED
-    inc = _device.to_device(increments, dt)
-    if inc.ndim != 2 or inc.shape[1] != width:
-        inc = inc.reshape(-1, width)
+    inc = _rows_of(_device.to_device(increments, dt), width)
+    _check_host_offsets(offsets, int(inc.shape[0]))
     off = _device.to_device(np.asarray(offsets, dtype=np.int64) if isinstance(offsets, (list, tuple))
                             else offsets)
     t = _device.torch()
@@ -300,9 +370,11 @@ def log_signature_batch(increments, offsets, width: int, depth: int, dtype=None,
     code = _dtype_code(dt)
     lie_basis = lie_basis or LieBasis(width, depth)
     t2l = lie_basis.get_t2l_matrix(np.float64)
-    inc = _device.to_device(increments, dt).reshape(-1, width)
+    inc = _rows_of(_device.to_device(increments, dt), width)
+    _check_host_offsets(offsets, int(inc.shape[0]))
     t = _device.torch()
-    off = _device.to_device(np.asarray(offsets, dtype=np.int64)).to(t.int64).contiguous()
+    off = _device.to_device(offsets if _device.is_device_array(offsets)
+                            else np.asarray(offsets, dtype=np.int64)).to(t.int64).contiguous()
     n = int(off.shape[0]) - 1
     out = _device.empty((n, lie_basis.size()), np.float64)
     _capi.check(_capi.lib().rpy_stream_log_signature_batch(
@@ -349,8 +421,11 @@ class DyadicSignatureCache:
         self.width, self.depth = int(width), int(depth)
         self.resolution, self.levels = int(resolution), int(levels)
         t = _device.torch()
-        self._inc = _device.to_device(increments, dt).reshape(-1, self.width)
+        self._inc = _rows_of(_device.to_device(increments, dt), self.width)
+        _check_host_offsets(offsets, int(self._inc.shape[0]))
         self._params = _device.to_device(params).to(t.float64).contiguous()
+        if int(self._params.shape[0]) != int(self._inc.shape[0]):
+            raise ValueError("params must hold one value per increment row")
         self._offs = _device.to_device(np.asarray(offsets, dtype=np.int64)
                                        if isinstance(offsets, (list, tuple, np.ndarray))
                                        else offsets).to(t.int64).contiguous()
@@ -416,11 +491,24 @@ class DyadicSignatureCache:
         """Stored signature of stream `stream` over [k 2^-power, (k+1) 2^-power)."""
         if not (0 <= power <= self.levels and 0 <= k < (1 << power)):
             raise IndexError("node outside the stored levels")
+        if not 0 <= stream < self.n_streams:
+            raise IndexError(f"stream {stream} outside 0..{self.n_streams - 1}")
         off = ((1 << power) - 1) * self.n_streams + stream * (1 << power) + k
         return self.cache[off * self.tsize:(off + 1) * self.tsize]
 
     def query(self, stream_ids, inf, sup, out=None):
+        """Signatures of [inf[q], sup[q]) on stream stream_ids[q].
+
+        Stream ids given on the host must lie in 0..n_streams-1 (IndexError
+        otherwise); ids already on the device are not checked, and one
+        outside that range answers the unit tensor, like an empty query.
+        `out`, when given, must be a contiguous device tensor of shape
+        (queries, tensor_size) and the cache's dtype."""
         t = _device.torch()
+        if not _device.is_device_array(stream_ids):
+            ids = np.asarray(stream_ids, dtype=np.int64).reshape(-1)
+            if ids.size and (ids.min() < 0 or ids.max() >= self.n_streams):
+                raise IndexError(f"stream ids must lie in 0..{self.n_streams - 1}")
         sid = _device.to_device(np.asarray(stream_ids, dtype=np.int64)
                                 if not _device.is_device_array(stream_ids) else stream_ids)
         sid = sid.to(t.int64).contiguous()
@@ -433,6 +521,16 @@ class DyadicSignatureCache:
             raise ValueError("stream_ids, inf and sup must have the same length")
         if out is None:
             out = _device.empty((nq, self.tsize), self.dtype)
+        else:
+            if not _device.is_device_array(out):
+                raise TypeError("out must be a device tensor")
+            if tuple(out.shape) != (nq, self.tsize):
+                raise ValueError(f"out must have shape {(nq, self.tsize)}, "
+                                 f"not {tuple(out.shape)}")
+            if _device.np_dtype(out) != self.dtype:
+                raise TypeError(f"out must have dtype {self.dtype}, not {_device.np_dtype(out)}")
+            if not out.is_contiguous() or out.device != self.cache.device:
+                raise ValueError("out must be contiguous and on the cache's device")
         _capi.check(_capi.lib().rpy_dyadic_cache_query(
             out.data_ptr(), self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
             self._offs.data_ptr(), self.n_streams, self.width, self.depth, self.resolution,
@@ -458,19 +556,40 @@ class LieIncrementStream:
 
     @classmethod
     def from_increments(cls, data, indices=None, *, width=None, depth=None, resolution=None,
-                        dtype=None, coeffs=None, support=None, ctx=None, **kwargs):
-        """Mirror of LieIncrementStream.from_increments (lie_increment_stream.cpp:73-245).
+                        dtype=None, coeffs=None, support=None, ctx=None, vtype=None,
+                        include_time=False, schema=None, channel_types=None, **kwargs):
+        """Mirror of LieIncrementStream.from_increments
+        (roughpy/src/streams/lie_increment_stream.cpp:73-245, keywords as
+        roughpy/src/args/kwargs_to_path_metadata.cpp:42-219).
 
-        Default indices are 0..N-1 with resolution 0; given indices without a
-        resolution use param_to_resolution(min index) + 1 (the reference's
-        running-minimum quirk).  Support is [min, max + 2^-resolution).  On a
-        repeated index the first row is kept.
+        * Without ctx, width defaults to the data's column count and depth
+          must be given: otherwise ValueError (:103-109).
+        * indices: None gives 0..N-1 with resolution 0; an int names a data
+          column read as cumulative parameter steps, the column staying part
+          of the increments (:133-168); an array or sequence gives the
+          parameters.  Given indices without a resolution use
+          py_param_to_resolution(min index) + 1 (the reference's
+          running-minimum quirk, :193-206).
+        * include_time appends a channel holding each row's parameter
+          increment (index minus the previous row's index, starting from 0;
+          streams/src/lie_increment_stream.cpp:135-159), so the width grows
+          by one.
+        * support replaces the support [min, max + 2^-resolution)
+          (Stream::restrict_to, streams/src/stream.cpp:73-80); it must be a
+          RealInterval.
+        * vtype ("dense" or "sparse") selects the reference's vector storage
+          and leaves the values unchanged; schema / channel_types are
+          accepted when they describe only increment channels.
+        On a repeated index the first row is kept.
         """
         if kwargs:
             raise TypeError(f"unexpected keyword arguments {sorted(kwargs)}")
         if ctx is not None:
             width = width if width is not None else ctx.width
             depth = depth if depth is not None else ctx.depth
+            if dtype is None:
+                dtype = getattr(ctx, "dtype", None)
+        _check_vtype(vtype)
         if isinstance(data, (list, tuple)) and data and hasattr(data[0], "data"):
             data = np.stack([np.asarray(getattr(d, "data")) for d in data])
         arr = np.asarray(data)
@@ -483,19 +602,37 @@ class LieIncrementStream:
             arr = np.zeros((0, int(width or 0)), dtype=dtype)
         if arr.ndim == 1:
             arr = arr.reshape(1, -1)
+        n = arr.shape[0]
+        n_schema = _schema_increment_channels(schema, channel_types)
         if width is None:
-            width = arr.shape[1]
+            width = n_schema if n_schema is not None else arr.shape[1]
         width = int(width)
+        if n_schema is not None and n_schema != width:
+            raise ValueError("specified width does not match the schema width")
+        if width == 0 or not depth:
+            raise ValueError("either ctx or both width and depth must be specified")
+        depth = int(depth)
         if arr.shape[1] < width:
             raise ValueError(f"data has {arr.shape[1]} columns, width {width} requested")
-        arr = np.ascontiguousarray(arr[:, :width], dtype=dtype)
-        depth = int(depth if depth is not None else 2)
-        n = arr.shape[0]
         if indices is None:
             idx = np.arange(n, dtype=np.float64)
-            res = 0
-        else:
+        elif isinstance(indices, (int, np.integer)) and not isinstance(indices, bool):
+            icol = int(indices)
+            if icol < 0 or icol >= arr.shape[1]:
+                raise RuntimeError("cannot find index column in provided data")
+            # running sum of the column (the first row's value is its index)
+            idx = np.cumsum(arr[:, icol].astype(np.float64))
+        elif isinstance(indices, (np.ndarray, list, tuple)) or hasattr(indices, "__array__"):
             idx = np.asarray(indices, dtype=np.float64).reshape(-1)
+        else:
+            raise TypeError("unexpected type provided to 'indices' argument")
+        if idx.shape[0] == 0:
+            # empty indices are the default ones (:170-175)
+            idx = np.arange(n, dtype=np.float64)
+            indices = None
+        if indices is None:
+            res = 0  # default indices force resolution 0 whatever was asked
+        else:
             if idx.shape[0] != n:
                 raise ValueError("mismatch between number of rows in data and number of indices")
             res = resolution
@@ -504,22 +641,25 @@ class LieIncrementStream:
                 for v in idx:  # `previous` stays 0 in the reference
                     if v - 0.0 < min_diff:
                         min_diff = v - 0.0
-                res = param_to_resolution(min_diff) + 1
-        # default indices force resolution 0 whatever was asked
-        # (roughpy/src/streams/lie_increment_stream.cpp:176-182)
+                res = py_param_to_resolution(min_diff) + 1
         res = int(res if res is not None else 0)
+        rows = np.ascontiguousarray(arr[:, :width], dtype=dtype)
+        if include_time:
+            prev = np.concatenate([[0.0], idx[:-1]]) if n else idx
+            rows = np.concatenate([rows, (idx - prev).astype(dtype)[:, None]], axis=1)
+            width += 1
         # first occurrence wins on a repeated index; rows sorted by index
         uniq, first = np.unique(idx, return_index=True)
         params = uniq.astype(np.float64)
-        rows = arr[first] if n else arr
+        rows = np.ascontiguousarray(rows[first]) if n else rows
         if n:
-            sup_ = float(idx.max()) + math.ldexp(1.0, -res)
-            sup_support = (float(idx.min()), sup_)
+            sup_support = (float(idx.min()), float(idx.max()) + math.ldexp(1.0, -res))
         else:
             sup_support = (0.0, 0.0)
         if support is not None:
-            sup_support = (float(support[0]), float(support[1])) if not isinstance(
-                support, RealInterval) else (support.inf, support.sup)
+            if not isinstance(support, RealInterval):
+                raise TypeError("Support must be an Interval")
+            sup_support = (support.inf, support.sup)
         return cls(params, rows, width, depth, res, sup_support, dtype)
 
     def __reduce__(self):
@@ -683,8 +823,12 @@ class LieIncrementStream:
         idx, offs = self._select_batch(infs, sups, res, self._sel_buf)
         if self._dev_rows is None:
             self._dev_rows = _device.to_device(np.ascontiguousarray(self._rows), self.dtype)
-        # a blocking copy: the buffer is rewritten by the next call
-        dev_idx = self._sel_pinned[:idx.shape[0]].to(self._dev_rows.device)
+        # a blocking copy: the buffer is rewritten by the next call.  The
+        # selection may have needed a larger array of its own; upload that one
+        if idx.base is self._sel_buf or idx is self._sel_buf:
+            dev_idx = self._sel_pinned[:idx.shape[0]].to(self._dev_rows.device)
+        else:
+            dev_idx = _device.to_device(np.ascontiguousarray(idx)).to(self._dev_rows.device)
         rows = self._dev_rows.index_select(0, dev_idx)
         lb = LieBasis(self.width, depth)
         lies = log_signature_batch(rows, offs, self.width, depth, dtype=self.dtype, lie_basis=lb)

@@ -15,7 +15,8 @@ from numpy.testing import assert_allclose, assert_array_equal
 import roughpy_amd as ra
 from oracle import oracle as O
 from roughpy_amd import _capi
-from roughpy_amd.streams import LieIncrementStream, param_to_resolution, to_dyadic_intervals
+from roughpy_amd.streams import (LieIncrementStream, RealInterval, param_to_resolution,
+                                 to_dyadic_intervals)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_goldens.json")))
@@ -101,10 +102,34 @@ def test_tensor_basis_mirror():
     assert list(tb.degree_begin) == [0, 1, 4, 13, 40, 121]
     assert tb.size() == 121
     assert tb.truncate(2) == ra.TensorBasis(3, 2)
+    assert tb.truncate(0) == ra.TensorBasis(3, 0)
+    # tensor_basis.c:177-191: new_depth must be in [0, depth)
+    for bad in (4, 5, -1):
+        with pytest.raises(ValueError):
+            tb.truncate(bad)
+    assert repr(tb) == "TensorBasis(3, 4)"
     assert tb != ra.TensorBasis(4, 4)
     assert len({ra.TensorBasis(2, 2), ra.TensorBasis(2, 2), ra.TensorBasis(2, 3)}) == 2
     with pytest.raises(ValueError):
         ra.TensorBasis(3, 4, degree_begin=[0, 1, 4, 13, 40, 120])
+
+
+def test_tensor_basis_hash_is_fnv1a():
+    """hash(TensorBasis) is the 64-bit FNV-1a of tensor_basis.c:144-171: the
+    FNV function against published test vectors, then the hash of the
+    reference's byte string (b"TensorBasis", int32 width, int32 depth,
+    depth + 2 pointer-sized degree_begin entries)."""
+    from roughpy_amd.bases import fnv1a64
+    assert fnv1a64(b"") == 0xCBF29CE484222325
+    assert fnv1a64(b"a") == 0xAF63DC4C8601EC8C
+    assert fnv1a64(b"foobar") == 0x85944171F73967E8
+    for w, d in [(2, 2), (5, 5), (4, 6)]:
+        tb = ra.TensorBasis(w, d)
+        raw = (b"TensorBasis" + np.int32(w).tobytes() + np.int32(d).tobytes()
+               + np.array(tb.degree_begin, dtype=np.intp).tobytes())
+        v = fnv1a64(raw)
+        assert hash(tb) == (v - (1 << 64) if v >= (1 << 63) else v)
+    assert hash(ra.TensorBasis(5, 5)) != hash(ra.TensorBasis(5, 4))
 
 
 @pytest.mark.parametrize("case", GOLD["to_dyadic_intervals"]["cases"])
@@ -240,28 +265,38 @@ def test_error_codes_map_to_exceptions():
     tb = _capi.TensorBasisC(0, 2, None)
     rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
                             _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
-                            _capi.DegreeBoundsC(0, -1), 1, 1, 1, 1, None)
+                            _capi.DegreeBoundsC(0, -1), 1, 1, 1, 1, 1, 1, 1, None)
     assert rc == _capi.RPY_ERR_BASIS
     with pytest.raises(ValueError, match="width"):
         _capi.check(rc)
     tb = _capi.TensorBasisC(2, 2, None)
     rc = L.rpy_dense_ft_fma(None, None, None, 7, ctypes.byref(tb), _capi.DegreeBoundsC(0, -1),
                             _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1), 1, 7, 7, 7,
-                            None)
+                            1, 1, 1, None)
     assert rc == _capi.RPY_ERR_DTYPE
     rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
                             _capi.DegreeBoundsC(2, 1), _capi.DegreeBoundsC(0, -1),
-                            _capi.DegreeBoundsC(0, -1), 1, 7, 7, 7, None)
+                            _capi.DegreeBoundsC(0, -1), 1, 7, 7, 7, 1, 1, 1, None)
     assert rc == _capi.RPY_ERR_INVALID_ARGUMENT
     rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
                             _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
-                            _capi.DegreeBoundsC(0, -1), 4, 6, 7, 7, None)
+                            _capi.DegreeBoundsC(0, -1), 4, 6, 7, 7, 1, 1, 1, None)
     assert rc == _capi.RPY_ERR_SHAPE
     # zero-size batch is a no-op that touches no pointer
     rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
                             _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
-                            _capi.DegreeBoundsC(0, -1), 0, 7, 7, 7, None)
+                            _capi.DegreeBoundsC(0, -1), 0, 7, 7, 7, 1, 1, 1, None)
     assert rc == _capi.RPY_OK
+    # a zero element stride is a shape error; a negative one is accepted
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 2, 7, 7, 7, 1, 0, 1, None)
+    assert rc == _capi.RPY_ERR_SHAPE
+    # an output whose elements would overlap under its strides
+    rc = L.rpy_dense_ft_fma(None, None, None, _capi.RPY_F64, ctypes.byref(tb),
+                            _capi.DegreeBoundsC(0, -1), _capi.DegreeBoundsC(0, -1),
+                            _capi.DegreeBoundsC(0, -1), 3, 2, 7, 7, 2, 1, 1, None)
+    assert rc == _capi.RPY_ERR_SHAPE
 
 
 def test_query_kernel_dyadic_cover_matches_reference_dissection(rng):
@@ -336,3 +371,90 @@ def test_select_rows_batch_threaded_blocks():
     bad_a[5000] = np.nan
     with pytest.raises(ValueError, match="finite"):
         st._select_batch(bad_a, qb, 2)
+
+
+# --- LieIncrementStream.from_increments keyword surface (host side) ----------
+
+def _meta(s):
+    return (s.width, s.depth, s.resolution, tuple(s.support), s._params.tolist(),
+            s._rows.tolist())
+
+
+def test_from_increments_requires_depth():
+    """lie_increment_stream.cpp:103-109: without ctx, width and depth must be
+    known (width defaults to the column count)."""
+    with pytest.raises(ValueError, match="width and depth"):
+        LieIncrementStream.from_increments(np.ones((3, 2)))
+    with pytest.raises(ValueError, match="width and depth"):
+        LieIncrementStream.from_increments(np.ones((3, 2)), width=2, depth=0)
+    s = LieIncrementStream.from_increments(np.ones((3, 2)), depth=3)
+    assert (s.width, s.depth, s.resolution) == (2, 3, 0)
+
+
+def test_from_increments_index_column_is_cumulative(rng):
+    """indices=<int> reads that column as parameter steps whose running sum
+    gives the indices (lie_increment_stream.cpp:133-168); the column stays a
+    data channel."""
+    data = np.abs(rng.normal(size=(7, 3))) + 0.1
+    by_col = LieIncrementStream.from_increments(data, indices=1, depth=2)
+    explicit = LieIncrementStream.from_increments(data, indices=np.cumsum(data[:, 1]), depth=2)
+    assert _meta(by_col) == _meta(explicit)
+    assert by_col.width == 3
+    with pytest.raises(RuntimeError):
+        LieIncrementStream.from_increments(data, indices=3, depth=2)
+    with pytest.raises(TypeError):
+        LieIncrementStream.from_increments(data, indices="0", depth=2)
+    with pytest.raises(ValueError, match="mismatch"):
+        LieIncrementStream.from_increments(data, indices=[0.0, 1.0], depth=2)
+
+
+def test_from_increments_inferred_resolution_matches_oracle():
+    """Given indices without a resolution: py_param_to_resolution(min index) + 1
+    (kwargs_to_path_metadata.cpp:224-229, lie_increment_stream.cpp:193-206),
+    negative when every index is 2 or more; support [min, max + 2^-res)."""
+    from roughpy_amd.streams import py_param_to_resolution
+    for v, want in [(0.7, 1), (1.0, 0), (2.0, -1), (3.0, -1), (16.0, -4), (40.0, -5),
+                    (1e6, -5), (0.1, 4)]:
+        assert py_param_to_resolution(v) == want, v
+    data = np.arange(12.0).reshape(4, 3)
+    for idx in ([4.0, 5.0, 6.5, 9.0], [0.3, 0.5, 0.55, 2.0], [40.0, 41.0, 50.0, 64.0]):
+        s = LieIncrementStream.from_increments(data, indices=np.array(idx), depth=2)
+        o = O.IncrementStream(data, 3, 2, indices=np.array(idx))
+        assert s.resolution == o.resolution
+        assert tuple(s.support) == tuple(o.support)
+
+
+def test_to_dyadic_intervals_negative_tolerance():
+    """Pieces longer than 1 (tolerance below 0) dissect as the oracle's
+    restatement of dyadic_interval.cpp:191-228 does."""
+    for a, b, tol in [(4.0, 8.0, -1), (3.0, 17.0, -2), (0.0, 64.0, -5), (-6.0, 10.0, -1)]:
+        assert to_dyadic_intervals(a, b, tol) == O.to_dyadic_intervals(a, b, tol)
+
+
+def test_from_increments_include_time_vtype_schema_support(rng):
+    data = rng.normal(size=(5, 2))
+    idx = np.array([0.5, 1.0, 1.0, 2.5, 4.0])
+    s = LieIncrementStream.from_increments(data, indices=idx, depth=2, include_time=True)
+    # the time channel holds index - previous index (0 before the first row);
+    # the repeated index keeps its first row
+    steps = idx - np.concatenate([[0.0], idx[:-1]])
+    keep = [0, 1, 3, 4]
+    assert s.width == 3
+    assert_allclose(s._rows, np.hstack([data, steps[:, None]])[keep])
+    for vt in ("dense", "sparse", "Dense"):
+        LieIncrementStream.from_increments(data, depth=2, vtype=vt)
+    with pytest.raises(TypeError):
+        LieIncrementStream.from_increments(data, depth=2, vtype="banded")
+    t = LieIncrementStream.from_increments(data, depth=2, schema=["increment", "increment"])
+    assert t.width == 2
+    t = LieIncrementStream.from_increments(data, depth=2, channel_types={"a": "increment",
+                                                                          "b": "increment"})
+    assert t.width == 2
+    with pytest.raises(NotImplementedError):
+        LieIncrementStream.from_increments(data, depth=2, schema=["increment", "categorical"])
+    with pytest.raises(ValueError, match="schema width"):
+        LieIncrementStream.from_increments(data, width=2, depth=2, schema=["increment"])
+    u = LieIncrementStream.from_increments(data, depth=2, support=RealInterval(-1.0, 3.0))
+    assert tuple(u.support) == (-1.0, 3.0)
+    with pytest.raises(TypeError, match="Interval"):
+        LieIncrementStream.from_increments(data, depth=2, support=(0.0, 1.0))

@@ -134,7 +134,7 @@ def test_degree_bounds_and_torch_inputs(gpu, rng):
         _capi.check(_capi.lib().rpy_dense_ft_fma(
             out.data_ptr(), ta.data_ptr(), tbb.data_ptr(), _capi.RPY_F64, tb.c_struct(),
             _capi.DegreeBoundsC(*ob), _capi.DegreeBoundsC(*lb), _capi.DegreeBoundsC(*rb), 5, n,
-            n, n, _device.stream_handle()))
+            n, n, 1, 1, 1, _device.stream_handle()))
         ref = c.copy()
         O.ft_fma(ref, a, b, w, d, out_bounds=ob, lhs_bounds=lb, rhs_bounds=rb)
         _close(out.cpu().numpy(), ref, w, d, np.float64)

@@ -88,7 +88,7 @@ def test_fma_in_a_graph(gpu):
         c.copy_(c0)
         _capi.check(_capi.lib().rpy_dense_ft_fma(
             c.data_ptr(), a.data_ptr(), b.data_ptr(), _capi.RPY_F32, cs, B(0, -1), B(0, -1),
-            B(0, -1), batch, n, n, n, _device.stream_handle()))
+            B(0, -1), batch, n, n, n, 1, 1, 1, _device.stream_handle()))
 
     refill(99)
     _replays(torch, refill, run, lambda: c)

@@ -298,7 +298,7 @@ def test_stream_tick_accuracy_golden():
     g = GOLD["stream_tick_accuracy"]
     s = O.IncrementStream(np.array(g["data"]), 2, 2, indices=np.array(g["indices"]))
     for q in g["queries"]:
-        r = s.signature(q["inf"], q["sup"], O.param_to_resolution(q["resolution_param"]))
+        r = s.signature(q["inf"], q["sup"], O.py_param_to_resolution(q["resolution_param"]))
         if q["expected"] == "unit":
             assert_array_equal(r, np.eye(1, 7).ravel())
         else:
