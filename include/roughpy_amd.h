@@ -287,6 +287,23 @@ int rpy_stream_signature_batch(void *out, const void *increments,
                                int32_t width, int32_t depth, rpy_dtype dtype,
                                void *stream);
 
+/* rpy_stream_signature_batch with every temporary (length keys, the sort's
+ * scratch, the work-queue counter) in a fixed scratch array of 8 MiB per GPU
+ * instead of allocated memory: the call allocates nothing.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Same arguments, results and errors, plus RPY_ERR_SHAPE when
+ * the batch needs more scratch than the array holds (about 300,000 streams;
+ * the message states the bytes needed).  The array keeps the last call's
+ * contents and is shared by every call on that GPU: calls that overlap on
+ * one GPU (other streams, other host threads) race and must be ordered by
+ * the caller. */
+int rpy_stream_signature_batch_noalloc(void *out, const void *increments,
+                                       const int64_t *offsets,
+                                       int64_t n_streams, int32_t width,
+                                       int32_t depth, rpy_dtype dtype,
+                                       void *stream);
+
 /* Log-signatures in the Hall basis: t2l(log(signature)) per stream
  * (lite_context.h:899-902; Stream.log_signature, stream.cpp:107-123).
  * The log and the t2l projection run in float64 whatever `dtype` is; out is

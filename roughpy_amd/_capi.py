@@ -87,6 +87,8 @@ SIGNATURES = {
                                             _i64, _i64, _i64, _vp]),
     "rpy_stream_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
                                                   ctypes.c_int, _vp]),
+    "rpy_stream_signature_batch_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
+                                                          ctypes.c_int, _vp]),
     "rpy_stream_log_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
                                                       ctypes.c_int, _SM, _vp]),
     "rpy_param_to_resolution": (_i32, [_dbl]),

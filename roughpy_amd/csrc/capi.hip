@@ -319,6 +319,14 @@ int rpy_stream_signature_batch(void* out, const void* increments, const int64_t*
                                   stream);
 }
 
+int rpy_stream_signature_batch_noalloc(void* out, const void* increments,
+                                       const int64_t* offsets, int64_t n_streams, int32_t width,
+                                       int32_t depth, rpy_dtype dtype, void* stream)
+{
+    return stream_signature_batch_noalloc(out, increments, offsets, n_streams, width, depth,
+                                          dtype, stream);
+}
+
 int rpy_stream_log_signature_batch(double* out, const void* increments, const int64_t* offsets,
                                    int64_t n_streams, int32_t width, int32_t depth,
                                    rpy_dtype dtype, const rpy_sparse_matrix* t2l, void* stream)

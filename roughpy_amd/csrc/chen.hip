@@ -15,6 +15,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -87,7 +88,6 @@ void chen_persistent_kernel(
     __shared__ int32_t slot_stream[kSlots];
 
     const int lane = threadIdx.x;
-    const int64_t n_streams = n_items / NP;
     while (true) {
         uint32_t chunk = 0;
         if (lane == 0) chunk = atomicAdd(counter, 1u);
@@ -330,95 +330,171 @@ int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64)
     return dtype_is_f64 ? e->p64 : e->p32;
 }
 
+namespace {
+
+// Device workspace of one signature call: keys in/out, values in/out, the
+// queue counter and the radix sort's scratch (specialised shapes), or the
+// float64 running signatures of the generic kernel's compensated mode.
+struct SigPlan {
+    rpy_tensor_basis tb;
+    DevBasis b;
+    const Entry* e;
+    size_t sort_bytes;  // radix sort scratch
+    size_t k_bytes, v_bytes;
+    size_t total;       // bytes of workspace the call needs
+    int generic_grid;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Validates the arguments and sizes the workspace; *skip = true when there
+// is nothing to launch.
+int plan_signature(const void* out, const void* increments, const int64_t* begins,
+                   const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
+                   rpy_dtype dtype, hipStream_t st, SigPlan& p, bool* skip,
+                   int max_generic_grid = 4096)
+{
+    *skip = false;
+    if (dtype != RPY_F32 && dtype != RPY_F64 && dtype != RPY_F32_COMPENSATED)
+        return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    p.tb = rpy_tensor_basis{width, depth, nullptr};
+    int rc = make_basis(&p.tb, p.b);
+    if (rc) return rc;
+    if (depth < 1) return fail(RPY_ERR_INVALID_ARGUMENT, "signatures need depth >= 1");
+    if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
+    if (n_streams == 0) {
+        *skip = true;
+        return RPY_OK;
+    }
+    if (n_streams > INT32_MAX) return fail(RPY_ERR_SHAPE, "too many streams");
+    if (!out || !begins || !ends || !increments)
+        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    p.e = find_entry(width, depth);
+    p.sort_bytes = p.k_bytes = p.v_bytes = 0;
+    p.generic_grid = int(std::min<int64_t>(n_streams, max_generic_grid));
+    if (!p.e) {
+        p.total = dtype == RPY_F32_COMPENSATED
+                          ? align256(size_t(p.generic_grid) * size_t(p.b.size()) * 8)
+                          : 0;
+        return RPY_OK;
+    }
+    const size_t n = size_t(n_streams);
+    RPY_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, p.sort_bytes, (uint32_t*)nullptr,
+                                                 (uint32_t*)nullptr, (int32_t*)nullptr,
+                                                 (int32_t*)nullptr, n, 0, 32, st));
+    p.k_bytes = align256(n * sizeof(uint32_t));
+    p.v_bytes = align256(n * sizeof(int32_t));
+    p.total = 2 * p.k_bytes + 2 * p.v_bytes + 256 + align256(p.sort_bytes);
+    return RPY_OK;
+}
+
+// Launches the call on workspace `ws` (p.total bytes, device memory).
+int run_signature(void* out, const void* increments, const int64_t* begins, const int64_t* ends,
+                  int64_t n_streams, rpy_dtype dtype, const SigPlan& p, char* ws,
+                  hipStream_t st)
+{
+    const DevBasis& b = p.b;
+    if (!p.e) {
+        const int grid = p.generic_grid;
+        if (dtype == RPY_F64)
+            hipLaunchKernelGGL((chen_generic_kernel<double>), dim3(grid), dim3(256), 0, st,
+                               (double*)out, (const double*)increments, begins, ends, n_streams,
+                               b.size(), b, (double*)nullptr);
+        else if (dtype == RPY_F32)
+            hipLaunchKernelGGL((chen_generic_kernel<float>), dim3(grid), dim3(256), 0, st,
+                               (float*)out, (const float*)increments, begins, ends, n_streams,
+                               b.size(), b, (float*)nullptr);
+        else
+            hipLaunchKernelGGL((chen_generic_kernel<double, float>), dim3(grid), dim3(256), 0, st,
+                               (float*)out, (const float*)increments, begins, ends, n_streams,
+                               b.size(), b, (double*)ws);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess)
+            return fail(RPY_ERR_HIP, "signature launch failed: %s", hipGetErrorString(err));
+        return RPY_OK;
+    }
+    const size_t n = size_t(n_streams);
+    uint32_t* keys_in = (uint32_t*)ws;
+    uint32_t* keys_out = (uint32_t*)(ws + p.k_bytes);
+    int32_t* vals_in = (int32_t*)(ws + 2 * p.k_bytes);
+    int32_t* vals_out = (int32_t*)(ws + 2 * p.k_bytes + p.v_bytes);
+    uint32_t* counter = (uint32_t*)(ws + 2 * p.k_bytes + 2 * p.v_bytes);
+    void* sort_tmp = ws + 2 * p.k_bytes + 2 * p.v_bytes + 256;
+    size_t sort_bytes = p.sort_bytes;
+
+    // the length keys also clear the queue counter (a kernel, not a memset,
+    // so a captured graph replays it)
+    const int pb = 256;
+    hipLaunchKernelGGL(prep_lengths_kernel, dim3(uint32_t((n + pb - 1) / pb)), dim3(pb), 0, st,
+                       begins, ends, n_streams, keys_in, vals_in, counter);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return fail(RPY_ERR_HIP, "length prep launch failed: %s", hipGetErrorString(err));
+    // Longest-first order only matters when the chunks run in more than one
+    // round; a batch whose waves are all resident at once (a few per CU)
+    // keeps the input order and skips the sort's launches.
+    const int np = (dtype == RPY_F64 || dtype == RPY_F32_COMPENSATED) ? p.e->p64 : p.e->p32;
+    const int64_t lanes = int64_t(n_streams) * int64_t(ipow(p.tb.width, np));
+    const bool sort = (lanes + 63) / 64 > 1024;
+    if (sort) {
+        err = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys_in, keys_out, vals_in,
+                                             vals_out, n, 0, 32, st);
+        if (err != hipSuccess)
+            return fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
+    }
+    Launcher f = (dtype == RPY_F64) ? p.e->f64 : (dtype == RPY_F32) ? p.e->f32 : p.e->f32c;
+    return f(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b, counter, st);
+}
+
+// Scratch of the entry point that allocates nothing: one array per GPU,
+// resolved to a device address once per device (its name is only a host
+// stand-in).  Calls that overlap on one GPU share it and race.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int scratch_pointer(char** out)
+{
+    int dev = 0;
+    RPY_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(RPY_ERR_UNSUPPORTED, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr)
+            return fail(RPY_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(err));
+        g_scratch_ptr[dev] = static_cast<char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return RPY_OK;
+}
+
+}  // namespace
+
 int stream_signature_ranges(void* out, const void* increments, const int64_t* begins,
                             const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
                             rpy_dtype dtype, void* stream)
 {
-    if (dtype != RPY_F32 && dtype != RPY_F64 && dtype != RPY_F32_COMPENSATED)
-        return fail(RPY_ERR_DTYPE, "unsupported dtype");
-    rpy_tensor_basis tb{width, depth, nullptr};
-    DevBasis b;
-    int rc = make_basis(&tb, b);
-    if (rc) return rc;
-    if (depth < 1) return fail(RPY_ERR_INVALID_ARGUMENT, "signatures need depth >= 1");
-    if (n_streams < 0) return fail(RPY_ERR_SHAPE, "negative number of streams");
-    if (n_streams == 0) return RPY_OK;
-    if (n_streams > INT32_MAX) return fail(RPY_ERR_SHAPE, "too many streams");
-    if (!out || !begins || !ends || !increments)
-        return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-
-    const Entry* e = find_entry(width, depth);
-    if (!e) {
-        const int grid = int(std::min<int64_t>(n_streams, 4096));
-        if (dtype == RPY_F64) {
-            hipLaunchKernelGGL((chen_generic_kernel<double>), dim3(grid), dim3(256), 0, st,
-                               (double*)out, (const double*)increments, begins, ends, n_streams,
-                               b.size(), b, (double*)nullptr);
-        } else if (dtype == RPY_F32) {
-            hipLaunchKernelGGL((chen_generic_kernel<float>), dim3(grid), dim3(256), 0, st,
-                               (float*)out, (const float*)increments, begins, ends, n_streams,
-                               b.size(), b, (float*)nullptr);
-        } else {
-            double* work = nullptr;
-            RPY_HIP_CHECK(workspace_alloc((void**)&work, size_t(grid) * size_t(b.size()) * 8, st));
-            hipLaunchKernelGGL((chen_generic_kernel<double, float>), dim3(grid), dim3(256), 0, st,
-                               (float*)out, (const float*)increments, begins, ends, n_streams,
-                               b.size(), b, work);
-            hipError_t err = hipGetLastError();
-            hipError_t ferr = hipFreeAsync(work, st);
-            if (err != hipSuccess)
-                return fail(RPY_ERR_HIP, "signature launch failed: %s", hipGetErrorString(err));
-            RPY_HIP_CHECK(ferr);
-            return RPY_OK;
-        }
-        RPY_HIP_CHECK(hipGetLastError());
-        return RPY_OK;
-    }
-
-    // workspace: keys in/out, values in/out, counter, radix-sort scratch
-    const size_t n = size_t(n_streams);
-    size_t sort_bytes = 0;
-    RPY_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, (uint32_t*)nullptr,
-                                                 (uint32_t*)nullptr, (int32_t*)nullptr,
-                                                 (int32_t*)nullptr, n, 0, 32, st));
-    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
-    const size_t k_bytes = align(n * sizeof(uint32_t));
-    const size_t v_bytes = align(n * sizeof(int32_t));
-    const size_t total = 2 * k_bytes + 2 * v_bytes + 256 + align(sort_bytes);
+    SigPlan p;
+    bool skip = false;
+    int rc = plan_signature(out, increments, begins, ends, n_streams, width, depth, dtype, st, p,
+                            &skip);
+    if (rc || skip) return rc;
     char* ws = nullptr;
-    RPY_HIP_CHECK(workspace_alloc((void**)&ws, total, st));
-    uint32_t* keys_in = (uint32_t*)ws;
-    uint32_t* keys_out = (uint32_t*)(ws + k_bytes);
-    int32_t* vals_in = (int32_t*)(ws + 2 * k_bytes);
-    int32_t* vals_out = (int32_t*)(ws + 2 * k_bytes + v_bytes);
-    uint32_t* counter = (uint32_t*)(ws + 2 * k_bytes + 2 * v_bytes);
-    void* sort_tmp = ws + 2 * k_bytes + 2 * v_bytes + 256;
-
-    const int pb = 256;
-    hipLaunchKernelGGL(prep_lengths_kernel, dim3(uint32_t((n + pb - 1) / pb)), dim3(pb), 0, st,
-                       begins, ends, n_streams, keys_in, vals_in, counter);
-    rc = RPY_OK;
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "length prep launch failed: %s", hipGetErrorString(err));
-    // Longest-first order only matters when the chunks run in more than one
-    // round; a batch whose waves are all resident at once (a few per CU)
-    // keeps the input order and skips the sort's launches.
-    const int np = (dtype == RPY_F64 || dtype == RPY_F32_COMPENSATED) ? e->p64 : e->p32;
-    const int64_t lanes = int64_t(n_streams) * int64_t(ipow(width, np));
-    const bool sort = (lanes + 63) / 64 > 1024;
-    if (!rc && sort) {
-        err = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys_in, keys_out, vals_in,
-                                             vals_out, n, 0, 32, st);
-        if (err != hipSuccess) rc = fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
+    if (p.total) RPY_HIP_CHECK(workspace_alloc((void**)&ws, p.total, st));
+    rc = run_signature(out, increments, begins, ends, n_streams, dtype, p, ws, st);
+    if (ws) {
+        hipError_t ferr = hipFreeAsync(ws, st);
+        if (!rc && ferr != hipSuccess)
+            rc = fail(RPY_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ferr));
     }
-    if (!rc) {
-        Launcher f = (dtype == RPY_F64) ? e->f64 : (dtype == RPY_F32) ? e->f32 : e->f32c;
-        rc = f(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b, counter,
-               st);
-    }
-    hipError_t ferr = hipFreeAsync(ws, st);
-    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ferr));
     return rc;
 }
 
@@ -430,6 +506,37 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
     // CSR offsets: stream s owns rows [offsets[s], offsets[s+1])
     return stream_signature_ranges(out, increments, offsets, offsets ? offsets + 1 : nullptr,
                                    n_streams, width, depth, dtype, stream);
+}
+
+int64_t stream_signature_scratch_bytes() { return int64_t(kScratchBytes); }
+
+int stream_signature_batch_noalloc(void* out, const void* increments, const int64_t* offsets,
+                                   int64_t n_streams, int32_t width, int32_t depth,
+                                   rpy_dtype dtype, void* stream)
+{
+    if (!offsets && n_streams > 0) return fail(RPY_ERR_INVALID_ARGUMENT, "null offsets");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int64_t* begins = offsets;
+    const int64_t* ends = offsets ? offsets + 1 : nullptr;
+    SigPlan p;
+    bool skip = false;
+    // the generic kernel's compensated slots must fit the scratch as well
+    int64_t tsize = tensor_size_checked(width, depth);
+    int max_grid = 4096;
+    if (dtype == RPY_F32_COMPENSATED && tsize > 0)
+        max_grid = int(std::max<int64_t>(1, std::min<int64_t>(4096, int64_t(kScratchBytes) /
+                                                                        (tsize * 8))));
+    int rc = plan_signature(out, increments, begins, ends, n_streams, width, depth, dtype, st, p,
+                            &skip, max_grid);
+    if (rc || skip) return rc;
+    if (p.total > kScratchBytes)
+        return fail(RPY_ERR_SHAPE,
+                    "%lld streams need %zu bytes of scratch, more than the %zu this entry point "
+                    "holds; use rpy_stream_signature_batch", (long long)n_streams, p.total,
+                    kScratchBytes);
+    char* ws = nullptr;
+    if ((rc = scratch_pointer(&ws))) return rc;
+    return run_signature(out, increments, begins, ends, n_streams, dtype, p, ws, st);
 }
 
 // Runs the lane algorithm of the specialised kernel on the host for one

@@ -19,6 +19,14 @@ int stream_signature_batch(void* out, const void* increments, const int64_t* off
                            int64_t n_streams, int32_t width, int32_t depth, rpy_dtype dtype,
                            void* stream);
 
+// The same with every temporary in a fixed per-device scratch array; no
+// allocation.  RPY_ERR_SHAPE when the batch needs more scratch than it holds.
+int stream_signature_batch_noalloc(void* out, const void* increments, const int64_t* offsets,
+                                   int64_t n_streams, int32_t width, int32_t depth,
+                                   rpy_dtype dtype, void* stream);
+// Bytes of that scratch array.
+int64_t stream_signature_scratch_bytes();
+
 // Same with explicit row ranges: stream s owns rows [begins[s], ends[s]).
 int stream_signature_ranges(void* out, const void* increments, const int64_t* begins,
                             const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
