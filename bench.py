@@ -13,6 +13,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -202,7 +203,7 @@ def fma_leg(torch, steps, warmup):
         def fn():
             _capi.check(L.rpy_dense_ft_fma(C.data_ptr(), A.data_ptr(), B.data_ptr(), code,
                                            cs, Bd(0, -1), Bd(0, -1), Bd(0, -1),
-                                           batch, n, n, n, _device.stream_handle()))
+                                           batch, n, n, n, 1, 1, 1, _device.stream_handle()))
         try:
             fn()
         except NotImplementedError:
@@ -227,25 +228,36 @@ def fma_leg(torch, steps, warmup):
 def dense_leg(torch, steps, warmup):
     """The other dense operations of the path on a batch of 16384 tensors of
     width 5, depth 5 (3906 coefficients; Lie dimension 829), inputs resident
-    in HBM, through the Python mirror of roughpy.compute: in-place product,
-    exp, fmexp, log, antipode, adjoint of left multiplication, lie_to_tensor
-    and tensor_to_lie.  Each is HBM-bound; the algorithmic bytes per element
-    are the operands read plus the result written."""
+    in HBM: in-place product, exp, fmexp, log, antipode, adjoint of left
+    multiplication, lie_to_tensor and tensor_to_lie.  Each is HBM-bound; the
+    algorithmic bytes per element are the operands read plus the result
+    written.  `ms` times the C-ABI call alone on preallocated outputs (HIP
+    events on the launch stream); `api_ms` times the same operation through
+    the Python mirror of roughpy.compute, which allocates its result on every
+    call, so the difference is the host and allocation overhead of the API."""
     from math import factorial
 
+    from roughpy_amd import _capi, _device
     from roughpy_amd import compute as rc
     from roughpy_amd.bases import LieBasis, TensorBasis
     w, d, batch = 5, 5, 16384
     tb, lb = TensorBasis(w, d), LieBasis(w, d)
     n, nl = tb.size(), lb.size()
     db = tb.degree_begin
+    cs = tb.c_struct()
+    L = _capi.lib()
+    st = _device.stream_handle()
+    Bd = _capi.DegreeBoundsC
+    full = Bd(0, -1)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(SEED + 80)
     scale = torch.empty(n, device="cuda")
     for k in range(d + 1):
         scale[db[k]:db[k + 1]] = 1.0 / factorial(k)
-    res = {"workload": f"batch {batch}, width {w}, depth {d} ({n} coefficients, Lie dimension {nl})"}
+    res = {"workload": f"batch {batch}, width {w}, depth {d} ({n} coefficients, Lie dimension "
+                       f"{nl})"}
     for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        code = _capi.RPY_F64 if dt == torch.float64 else _capi.RPY_F32
         A = ((torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale).to(dt)
         B = ((torch.rand((batch, n), generator=gen, device="cuda") * 2 - 1) * scale).to(dt)
         A[:, 0] = 1
@@ -253,26 +265,99 @@ def dense_leg(torch, steps, warmup):
         X = B.clone()
         X[:, 0] = 0
         Lz = (torch.rand((batch, nl), generator=gen, device="cuda") * 2 - 1).to(dt)
+        O = torch.empty((batch, n), dtype=dt, device="cuda")
+        OL = torch.empty((batch, nl), dtype=dt, device="cuda")
+        l2t = lb.get_l2t_matrix(np.dtype("float64" if name == "f64" else "float32"))
+        t2l = lb.get_t2l_matrix(np.dtype("float64" if name == "f64" else "float32"))
         es = A.element_size()
-        ops = {
-            "inplace_mul": (lambda: rc.ft_inplace_mul(rc.FreeTensor(A, tb), rc.FreeTensor(B, tb)),
-                            3 * n),
-            "exp": (lambda: rc.ft_exp(rc.FreeTensor(X, tb)), 2 * n),
-            "fmexp": (lambda: rc.ft_fmexp(rc.FreeTensor(B, tb), rc.FreeTensor(X, tb)), 3 * n),
-            "log": (lambda: rc.ft_log(rc.FreeTensor(B, tb)), 2 * n),
-            "antipode": (lambda: rc.antipode(rc.FreeTensor(B, tb)), 2 * n),
-            "adjoint_left_mul": (lambda: rc.ft_adjoint_left_mul(rc.FreeTensor(B, tb),
-                                                                 rc.ShuffleTensor(X, tb)), 3 * n),
-            "lie_to_tensor": (lambda: rc.lie_to_tensor(rc.Lie(Lz, lb), tb), nl + n),
-            "tensor_to_lie": (lambda: rc.tensor_to_lie(rc.FreeTensor(B, tb), lb), n + nl),
+        P = lambda t: t.data_ptr()  # noqa: E731
+        kernels = {
+            "inplace_mul": (lambda: L.rpy_dense_ft_inplace_mul(P(A), P(B), code, cs, full, full,
+                                                               batch, n, n, 1, 1, st), 3 * n),
+            "exp": (lambda: L.rpy_dense_ft_exp(P(O), P(X), code, cs, full, full, batch, n, n, 1,
+                                               1, st), 2 * n),
+            "fmexp": (lambda: L.rpy_dense_ft_fmexp(P(O), P(B), P(X), code, cs, full, full, full,
+                                                   batch, n, n, n, 1, 1, 1, st), 3 * n),
+            "log": (lambda: L.rpy_dense_ft_log(P(O), P(B), code, cs, full, full, batch, n, n, 1,
+                                               1, st), 2 * n),
+            "antipode": (lambda: L.rpy_dense_ft_antipode(P(O), P(B), code, cs, full, full, batch,
+                                                         n, n, 1, 1, st), 2 * n),
+            "adjoint_left_mul": (lambda: L.rpy_dense_ft_adjoint_left_mul(
+                P(O), P(B), P(X), code, cs, full, full, full, batch, n, n, n, 1, 1, 1, st),
+                3 * n),
+            "lie_to_tensor": (lambda: L.rpy_dense_sparse_map(P(O), P(Lz), l2t.device_struct(),
+                                                             code, 1.0, batch, n, nl, 1, 1, st),
+                              nl + n),
+            "tensor_to_lie": (lambda: L.rpy_dense_sparse_map(P(OL), P(B), t2l.device_struct(),
+                                                             code, 1.0, batch, nl, n, 1, 1, st),
+                              n + nl),
+        }
+        api = {
+            "inplace_mul": lambda: rc.ft_inplace_mul(rc.FreeTensor(A, tb), rc.FreeTensor(B, tb)),
+            "exp": lambda: rc.ft_exp(rc.FreeTensor(X, tb)),
+            "fmexp": lambda: rc.ft_fmexp(rc.FreeTensor(B, tb), rc.FreeTensor(X, tb)),
+            "log": lambda: rc.ft_log(rc.FreeTensor(B, tb)),
+            "antipode": lambda: rc.antipode(rc.FreeTensor(B, tb)),
+            "adjoint_left_mul": lambda: rc.ft_adjoint_left_mul(rc.FreeTensor(B, tb),
+                                                               rc.ShuffleTensor(X, tb)),
+            "lie_to_tensor": lambda: rc.lie_to_tensor(rc.Lie(Lz, lb), tb),
+            "tensor_to_lie": lambda: rc.tensor_to_lie(rc.FreeTensor(B, tb), lb),
         }
         out = {}
-        for op, (fn, vals) in ops.items():
-            ms = time_kernel(fn, steps, warmup, torch)
+        for op, (fn, vals) in kernels.items():
+            def call(fn=fn):
+                _capi.check(fn())
+            ms = time_kernel(call, steps, warmup, torch)
+            api_ms = time_kernel(api[op], steps, warmup, torch)
             gbs = float(vals) * es * batch / (ms * 1e-3) / 1e9
-            out[op] = {"ms": ms, "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+            out[op] = {"ms": ms, "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+                       "api_ms": api_ms}
         res[name] = out
-        del A, B, X, Lz
+        del A, B, X, Lz, O, OL
+    torch.cuda.empty_cache()
+    return res
+
+
+def log_signature_leg(torch, steps, warmup):
+    """Log-signatures and their gradient at width 8, depth 4 (4681 tensor
+    coefficients, Lie dimension 1212), 4096 streams of length 256 with
+    N(0, 1/256) increments, loss <G, logsig> with a seeded G: forward (the
+    batched log-signature entry point) and backward (autograd through the
+    library's kernels), float32 and float64."""
+    from roughpy_amd import _capi, _device
+    from roughpy_amd import autograd as ag
+    from roughpy_amd.bases import LieBasis
+    w, d, n, L_ = 8, 4, 4096, 256
+    rng = np.random.default_rng(SEED + 32)
+    inc32 = torch.from_numpy(rng.normal(0.0, np.sqrt(1.0 / L_), size=(n * L_, w))
+                             .astype(np.float32)).cuda()
+    offs = torch.arange(0, n * L_ + 1, L_, dtype=torch.int64, device="cuda")
+    lb = LieBasis(w, d)
+    t2l = lb.get_t2l_matrix(np.float64)
+    G = torch.from_numpy(rng.normal(size=(n, lb.size()))).cuda()
+    res = {"workload": f"log-signatures, width {w}, depth {d} (Lie dimension {lb.size()}), "
+                       f"{n} streams of length {L_}"}
+    lib = _capi.lib()
+    for name, code, inc in (("f64", _capi.RPY_F64, inc32.double()), ("f32", _capi.RPY_F32, inc32)):
+        out = torch.empty((n, lb.size()), dtype=torch.float64, device="cuda")
+
+        def fwd():
+            _capi.check(lib.rpy_stream_log_signature_batch(
+                out.data_ptr(), inc.data_ptr(), offs.data_ptr(), n, w, d, code,
+                t2l.device_struct(), _device.stream_handle()))
+
+        x = inc.clone().requires_grad_(True)
+
+        def fwd_bwd():
+            x.grad = None
+            (ag.log_signature(x, offs, w, d) * G).sum().backward()
+
+        fwd_ms = time_kernel(fwd, steps, warmup, torch)
+        both_ms = time_kernel(fwd_bwd, max(1, steps // 2), 1, torch)
+        res[name] = {"forward_ms": fwd_ms, "forward_backward_ms": both_ms,
+                     "gradients_per_s": n / (both_ms * 1e-3)}
+        del out, x
+    del inc32, G
     torch.cuda.empty_cache()
     return res
 
@@ -403,14 +488,15 @@ def shuffle_leg(torch, steps, warmup):
 
         def fn():
             _capi.check(lib.rpy_dense_st_fma(C.data_ptr(), A.data_ptr(), B.data_ptr(), code, cs,
-                                             batch, n, n, n, _device.stream_handle()))
+                                             batch, n, n, n, 1, 1, 1, _device.stream_handle()))
         ms = time_kernel(fn, steps, warmup, torch)
         tf = 2.0 * terms * batch / (ms * 1e-3) / 1e12
         gbs = 4.0 * n * A.element_size() * batch / (ms * 1e-3) / 1e9
 
         def adj():
             _capi.check(lib.rpy_dense_st_adj_mul(C.data_ptr(), A.data_ptr(), B.data_ptr(), code,
-                                                 cs, batch, n, n, n, _device.stream_handle()))
+                                                 cs, batch, n, n, n, 1, 1, 1,
+                                                 _device.stream_handle()))
         ms_adj = time_kernel(adj, max(1, steps // 3), 1, torch)
         res[name] = {"ms_per_step": ms, "tflops": tf, "valu_frac": tf / PEAK_TFLOPS[name],
                      "hbm_gb_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
@@ -504,25 +590,40 @@ def serialize_leg(torch, inc, offs, lens, n_streams=256, levels=6, resolution=12
 
 
 def graph_leg(torch, reps=200):
-    """A launch-bound call: signatures of 16 short streams (width 5, depth 5,
-    lengths 2..8), timed as eager calls and as replays of a HIP graph
-    captured once (the length sort, the queue reset and the persistent
-    launch)."""
+    """A launch-bound multi-launch call: float64 log-signatures of 64 short
+    streams (width 5, depth 5, lengths 2..16) from entry points that
+    allocate nothing -- length keys and the persistent signature launch
+    (rpy_stream_signature_batch_noalloc), the tensor log and the Hall
+    projection on preallocated buffers -- timed as eager calls and as
+    replays of one HIP graph captured once."""
     from roughpy_amd import _capi, _device
-    w, d, n = 5, 5, 16
+    from roughpy_amd.bases import LieBasis, TensorBasis
+    w, d, n = 5, 5, 64
     rng = np.random.default_rng(SEED + 50)
-    lens = rng.integers(2, 9, size=n)
+    lens = rng.integers(2, 17, size=n)
     offs = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).cuda()
-    inc = torch.from_numpy((rng.normal(size=(int(lens.sum()), w)) * 0.1).astype(np.float32)).cuda()
-    tsize = int(_capi.lib().rpy_tensor_size(w, d))
-    out = torch.empty((n, tsize), device="cuda")
+    inc = torch.from_numpy(rng.normal(size=(int(lens.sum()), w)) * 0.1).cuda()
+    tb, lb = TensorBasis(w, d), LieBasis(w, d)
+    t2l = lb.get_t2l_matrix(np.float64)
+    tsize = tb.size()
+    sig = torch.empty((n, tsize), dtype=torch.float64, device="cuda")
+    lg = torch.empty((n, tsize), dtype=torch.float64, device="cuda")
+    out = torch.empty((n, lb.size()), dtype=torch.float64, device="cuda")
     lib = _capi.lib()
+    cs = tb.c_struct()
+    full = _capi.DegreeBoundsC(0, -1)
 
     def run():
-        _capi.check(lib.rpy_stream_signature_batch(out.data_ptr(), inc.data_ptr(),
-                                                   offs.data_ptr(), n, w, d, _capi.RPY_F32,
-                                                   _device.stream_handle()))
+        st = _device.stream_handle()
+        _capi.check(lib.rpy_stream_signature_batch_noalloc(
+            sig.data_ptr(), inc.data_ptr(), offs.data_ptr(), n, w, d, _capi.RPY_F64, st))
+        _capi.check(lib.rpy_dense_ft_log(lg.data_ptr(), sig.data_ptr(), _capi.RPY_F64, cs, full,
+                                         full, n, tsize, tsize, 1, 1, st))
+        out.zero_()
+        _capi.check(lib.rpy_dense_sparse_map(out.data_ptr(), lg.data_ptr(), t2l.device_struct(),
+                                             _capi.RPY_F64, 1.0, n, lb.size(), tsize, 1, 1, st))
     eager_ms = time_kernel(run, reps, 5, torch)
+    ref = out.clone()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -533,8 +634,11 @@ def graph_leg(torch, reps=200):
     with torch.cuda.graph(g):
         run()
     graph_ms = time_kernel(g.replay, reps, 5, torch)
-    return {"workload": f"signatures of {n} streams, width {w}, depth {d}, lengths 2..8",
-            "eager_us_per_call": eager_ms * 1e3, "graph_us_per_replay": graph_ms * 1e3}
+    same = bool(torch.equal(out, ref))
+    return {"workload": f"float64 log-signatures of {n} streams, width {w}, depth {d}, lengths "
+                        "2..16: length keys, signature, log, clearing and Hall projection launches",
+            "eager_us_per_call": eager_ms * 1e3, "graph_us_per_replay": graph_ms * 1e3,
+            "replay_equals_eager": same}
 
 
 def ref_cpu_rates(w, d, ops, target_seconds=0.4):
@@ -571,16 +675,46 @@ def ref_cpu_rates(w, d, ops, target_seconds=0.4):
     return out
 
 
-def cpu_baseline(inc, offs, target_seconds=10.0):
-    """Reference CPU path on a bounded sample: the reference's own
-    roughpy_compute ft_fmexp per increment (oracle/_ref/libref.so, built from
-    the reference headers), one stream per thread across the host cores."""
+def host_cpu():
+    """CPU model and the number of logical cores this process may use."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:  # lscpu may print only the family; /proc/cpuinfo names the part
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                full = line.split(":", 1)[1].strip()
+                if full and (model is None or len(full) > len(model)):
+                    model = full
+                break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def cpu_baseline(inc, offs, target_seconds=8.0):
+    """Reference CPU path on bounded samples of the same workload: the
+    reference's own roughpy_compute ft_fmexp per increment (oracle/_ref/
+    libref.so, built from the reference headers) -- on one thread, as
+    roughpy.compute runs (it holds the GIL), and with one stream per thread
+    on every host core this process may use, as GIL-free Stream.signature
+    calls would.  About target_seconds of wall time each."""
     from oracle import oracle as O
     use_ref = O.ref_lib() is not None
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, cores = host_cpu()
     rng = np.random.default_rng(SEED + 1)
 
-    def run(sample):
+    def run(sample, threads):
         lens = offs[1:][sample] - offs[:-1][sample]
         sub_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
         rows = np.concatenate([inc[offs[s]:offs[s + 1]] for s in sample])
@@ -588,19 +722,27 @@ def cpu_baseline(inc, offs, target_seconds=10.0):
         O.signature_batch(rows, sub_off, W, D, nthreads=threads, use_ref=use_ref)
         return time.perf_counter() - t0
 
-    calib = rng.choice(N_STREAMS, size=2 * threads, replace=False)
-    dt = run(calib)
-    per_stream = dt / len(calib)
-    n = int(min(N_STREAMS, max(len(calib), target_seconds / max(per_stream, 1e-9))))
-    sample = rng.choice(N_STREAMS, size=n, replace=False)
-    dt = run(sample)
+    def rate(threads):
+        calib = rng.choice(N_STREAMS, size=max(4, 2 * threads), replace=False)
+        per_stream = run(calib, threads) / len(calib)
+        n = int(min(N_STREAMS, max(len(calib), target_seconds / max(per_stream, 1e-9))))
+        sample = rng.choice(N_STREAMS, size=n, replace=False)
+        dt = run(sample, threads)
+        return n / dt, n, dt
+
+    v1, n1, t1 = rate(1)
+    vn, nn, tn = rate(cores)
+    kind = "reference" if use_ref else "port"
     return {
-        "value": n / dt,
+        "value": vn,
         "unit": "signatures/s",
-        "cores": threads,
-        "kind": "reference" if use_ref else "port",
-        "sample": f"{n} of the {N_STREAMS} streams (same length distribution), float32 data, "
-                  f"{dt:.1f} s wall on {threads} threads",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"{nn} of the {N_STREAMS} streams (same length distribution), float32 data, "
+                  f"{tn:.1f} s wall on {cores} threads (one stream per thread)",
+        "cpu_model": model,
+        "single_thread": {"value": v1, "unit": "signatures/s", "cores": 1, "kind": kind,
+                          "sample": f"{n1} streams, {t1:.1f} s on one thread"},
     }
 
 
@@ -612,10 +754,11 @@ def main():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--legs",
-                    default="dyadic,fma,dense,backward,gram,shuffle,simplify,serialize,graph",
+                    default="dyadic,fma,dense,backward,logsig,gram,shuffle,simplify,serialize,"
+                            "graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, dense, backward, gram, shuffle, simplify, serialize, "
-                         "graph; empty for none)")
+                         "(dyadic, fma, dense, backward, logsig, gram, shuffle, simplify, "
+                         "serialize, graph; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
                          "dispatch for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -724,6 +867,8 @@ def main():
         line["dense_w5_d5"] = dense_leg(torch, max(3, args.steps // 2), 1)
     if "backward" in legs:
         line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
+    if "logsig" in legs:
+        line["log_signature_w8_d4"] = log_signature_leg(torch, max(3, args.steps // 2), 1)
     if "gram" in legs:
         line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
     if "shuffle" in legs:
