@@ -15,6 +15,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -201,6 +203,151 @@ void chen_persistent_kernel(
     }
 }
 
+// Stream-per-workgroup variant for prefix splits of 64 lanes or more: a
+// workgroup of ceil(W^P / 64) waves takes one stream at a time from the
+// length-sorted queue, so every lane of it steps the same increment.  Rows
+// are staged block by block (kBlk steps, rows past the end of the stream
+// zero, which leaves the signature unchanged) into LDS as
+//   uni[t][a]            = x_a                      (read by every lane)
+//   tab[t][a][d - 1]     = x_a / d,  d = 1..D       (the lane's path letters)
+// and the block's steps are unrolled, so every LDS read takes its step's
+// offset as an immediate.  Subtree partials fold their 1/d (UniformRow), so
+// a step issues the lane's Horner FMAs and no scaling of the increment.
+template <typename T, int W, int D, int P>
+struct StreamCfg {
+    static constexpr int NP = int(ipow(W, P));
+    static constexpr int NW = (NP + 63) / 64;
+    static constexpr int NT = NW * 64;
+    static constexpr int LS = D % 2 ? D + 1 : D;    // slots per letter, even for b128 pairs
+    static constexpr int US = W % 2 ? W + 1 : W;    // uniform slots per step
+    static constexpr int STEP = W * LS + US;        // values per staged step
+    static constexpr int kBlk = 32;
+    static constexpr int kRaw = kBlk * W;           // raw values per block
+    static constexpr int kPer = (kRaw + NT - 1) / NT;
+};
+
+template <typename T, int W, int D, int P, typename IO = T>
+__global__ __launch_bounds__((StreamCfg<T, W, D, P>::NT))
+void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
+                        const int64_t* __restrict__ begins, const int64_t* __restrict__ ends,
+                        const int32_t* __restrict__ order, int64_t n_streams, int64_t tsize,
+                        DevBasis b, uint32_t* __restrict__ counter)
+{
+    using Sh = chen::Shape<W, D, P>;
+    using C = StreamCfg<T, W, D, P>;
+    constexpr int NP = C::NP, LS = C::LS, STEP = C::STEP, kBlk = C::kBlk;
+    __shared__ __align__(16) T stage[2][kBlk * STEP];
+    __shared__ int64_t s_row0, s_len;
+    __shared__ int32_t s_stream;
+
+    const int tid = threadIdx.x;
+    const bool active = tid < NP;
+    const int q = active ? tid : 0;
+    int u[P];
+    {
+        int t = q;
+        for (int j = P - 1; j >= 0; --j) {
+            u[j] = t % W;
+            t /= W;
+        }
+    }
+    T pre[C::kPer];
+    // raw value v of the block at step t0: row t0 + v / W, letter v % W
+    auto fetch = [&](int64_t row0, int64_t len, int64_t t0) {
+#pragma unroll
+        for (int i = 0; i < C::kPer; ++i) {
+            const int v = tid + C::NT * i;
+            const int r = v / W;
+            T val = T(0);
+            if (v < C::kRaw && t0 + r < len) val = T(incr[(row0 + t0) * W + v]);
+            pre[i] = val;
+        }
+    };
+    auto store = [&](T* st_) {
+#pragma unroll
+        for (int i = 0; i < C::kPer; ++i) {
+            const int v = tid + C::NT * i;
+            if (v < C::kRaw) {
+                const int r = v / W;
+                const int a = v - r * W;
+                T* step = st_ + r * STEP;
+                step[W * LS + a] = pre[i];
+                chen::static_for<D>([&](auto d0) {
+                    constexpr int d = int(decltype(d0)::value) + 1;
+                    step[a * LS + d - 1] = (d == 1) ? pre[i] : pre[i] * (T(1) / T(d));
+                });
+            }
+        }
+    };
+
+    while (true) {
+        __syncthreads();  // the previous stream is done with the slot values
+        if (tid == 0) {
+            const uint32_t idx = atomicAdd(counter, 1u);
+            int32_t s = -1;
+            int64_t r0 = 0, ln = 0;
+            if (int64_t(idx) < n_streams) {
+                s = order[idx];
+                r0 = begins[s];
+                ln = ends[s] - r0;
+                if (ln < 0) ln = 0;
+            }
+            s_stream = s;
+            s_row0 = r0;
+            s_len = ln;
+        }
+        __syncthreads();
+        const int32_t s = s_stream;
+        if (s < 0) break;
+        const int64_t row0 = s_row0, len = s_len;
+
+        chen::Lane<T, W, D, P> st;
+        st.init_identity();
+        fetch(row0, len, 0);
+        store(stage[0]);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t t0 = 0; t0 < len; t0 += kBlk) {
+            const bool more = t0 + kBlk < len;
+            if (more) fetch(row0, len, t0 + kBlk);
+            const T* blk = stage[buf];
+#pragma unroll 1
+            for (int tt = 0; tt < kBlk; ++tt) {
+                const T* step = blk + tt * STEP;
+                T raw[W];
+                chen::static_for<W>([&](auto a) { raw[a] = step[W * LS + int(decltype(a)::value)]; });
+                const T* lt[P];
+                chen::static_for<P>([&](auto j) {
+                    constexpr int jj = int(decltype(j)::value);
+                    lt[jj] = step + u[jj] * LS;
+                });
+                st.step(chen::UniformRow<T, W, D, P, LS>{raw, lt});
+                // keep each step's loads with its step: hoisting a whole
+                // unrolled block's reads runs out of registers
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (more) store(stage[buf ^ 1]);
+            __syncthreads();
+            buf ^= 1;
+        }
+
+        if (active) {
+            IO* o = out + int64_t(s) * tsize;
+            chen::static_for<D - P + 1>([&](auto i) {
+                constexpr int k = P + int(decltype(i)::value);
+                constexpr int nk = int(ipow(W, k - P));
+                IO* ok = o + b.db[k] + int64_t(q) * nk;
+                chen::static_for<nk>([&](auto v) { ok[v] = IO(st.sub[Sh::off(k) + v]); });
+            });
+            chen::static_for<P>([&](auto i) {
+                constexpr int k = int(decltype(i)::value);
+                constexpr int grp = int(ipow(W, P - k));
+                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.path[k]);
+            });
+        }
+    }
+}
+
 // Fallback for shapes without a specialised kernel: one workgroup per
 // stream, the running signature in the output buffer, per-coefficient Horner
 // with a barrier between levels.
@@ -278,23 +425,79 @@ int launch_persistent(void* out, const void* incr, const int64_t* begins, const 
     return RPY_OK;
 }
 
+template <typename T, int W, int D, int P, typename IO = T>
+int launch_stream(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
+                  const int32_t* order, int64_t n_streams, const DevBasis& b, uint32_t* counter,
+                  hipStream_t st)
+{
+    using C = StreamCfg<T, W, D, P>;
+    int dev = 0, cus = 0, per_cu = 0;
+    RPY_HIP_CHECK(hipGetDevice(&dev));
+    RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, chen_stream_kernel<T, W, D, P, IO>, C::NT, 0));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_streams, int64_t(cus) * per_cu));
+    hipLaunchKernelGGL((chen_stream_kernel<T, W, D, P, IO>), dim3(uint32_t(grid)), dim3(C::NT),
+                       0, st, (IO*)out, (const IO*)incr, begins, ends, order, n_streams, b.size(),
+                       b, counter);
+    RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
 using chen::choose_p;
 using chen::kBudgetF32;
 using chen::kBudgetF64;
 
+// Kernel variants of one shape: [0] the lane-chunk kernel (64 (stream,
+// prefix) items per wave), [1] the stream-per-workgroup kernel (prefix
+// splits of 64 lanes or more only, nullptr otherwise).
 struct Entry {
     int w, d, p32, p64;
-    Launcher f32, f64, f32c;
+    Launcher f32[2], f64[2], f32c[2];
 };
+
+template <typename T, int W, int D, int P, typename IO = T>
+constexpr Launcher stream_or_null()
+{
+    if constexpr (ipow(W, P) >= 64 && ipow(W, P) <= 1024)
+        return &launch_stream<T, W, D, P, IO>;
+    else
+        return nullptr;
+}
 
 template <int W, int D>
 Entry make_entry()
 {
     constexpr int P32 = choose_p(W, D, kBudgetF32);
     constexpr int P64 = choose_p(W, D, kBudgetF64);
-    return Entry{W, D, P32, P64, &launch_persistent<float, W, D, P32>,
-                 &launch_persistent<double, W, D, P64>,
-                 &launch_persistent<double, W, D, P64, float>};
+    return Entry{W, D, P32, P64,
+                 {&launch_persistent<float, W, D, P32>, stream_or_null<float, W, D, P32>()},
+                 {&launch_persistent<double, W, D, P64>, stream_or_null<double, W, D, P64>()},
+                 {&launch_persistent<double, W, D, P64, float>,
+                  stream_or_null<double, W, D, P64, float>()}};
+}
+
+// Which variant runs: the stream kernel where it exists, unless the
+// environment variable ROUGHPY_AMD_SIGNATURE_KERNEL is "chunk" (or "stream"
+// to insist), read once.
+int variant_override()
+{
+    static const int v = [] {
+        const char* e = std::getenv("ROUGHPY_AMD_SIGNATURE_KERNEL");
+        if (!e) return -1;
+        if (std::strcmp(e, "chunk") == 0) return 0;
+        if (std::strcmp(e, "stream") == 0) return 1;
+        return -1;
+    }();
+    return v;
+}
+
+Launcher pick(const Launcher (&f)[2])
+{
+    const int v = variant_override();
+    if (v == 0 || !f[1]) return f[0];
+    return f[1];
 }
 
 const std::vector<Entry>& table()
@@ -442,7 +645,8 @@ int run_signature(void* out, const void* increments, const int64_t* begins, cons
         if (err != hipSuccess)
             return fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
     }
-    Launcher f = (dtype == RPY_F64) ? p.e->f64 : (dtype == RPY_F32) ? p.e->f32 : p.e->f32c;
+    Launcher f = (dtype == RPY_F64) ? pick(p.e->f64) : (dtype == RPY_F32) ? pick(p.e->f32)
+                                                                         : pick(p.e->f32c);
     return f(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b, counter, st);
 }
 
@@ -556,12 +760,23 @@ static void emulate(double* out, const double* incr, int64_t len, const DevBasis
         chen::Lane<double, W, D, P> st;
         st.init_identity();
         double row[Sh::ROW];
+        // letter-major table of the stream kernel: tab[a * D + d - 1] = x_a / d
+        double tab[W * D];
         for (int64_t r = 0; r < len; ++r) {
             chen::build_row<double, W, D>(row, incr + r * W);
-            if (r % 2)
+            if (r % 3 == 1) {
                 st.step(chen::ScaledRow<double, W, D, P>{row, u});
-            else
+            } else if (r % 3 == 2) {
+                for (int a = 0; a < W; ++a)
+                    for (int d = 1; d <= D; ++d) tab[a * D + d - 1] = row[(d - 1) * W + a];
+                const double* lt[P];
+                for (int j = 0; j < P; ++j) lt[j] = tab + u[j] * D;
+                double raw[W];
+                for (int a = 0; a < W; ++a) raw[a] = incr[r * W + a];
+                st.step(chen::UniformRow<double, W, D, P, D>{raw, lt});
+            } else {
                 st.step(chen::RawRow<double, W, D, P>(row, u));
+            }
         }
         for (int k = P; k <= D; ++k) {
             const int nk = int(ipow(W, k - P));

@@ -36,6 +36,7 @@ namespace chen {
 // recovery step A <- A (x) exp(-x)).
 template <typename T, int W, int D, int P, bool Neg>
 struct RegRow {
+    static constexpr bool kFold = false;
     const T (&tab)[D * W];  // tab[(d-1) W + a] = x_a * (1/d)
     const T (&pu)[P];       // x_{u_j}
     template <int d>

@@ -110,6 +110,7 @@ RPY_HD inline void build_row(T* row, const T* x)
 // x_a / d read from the scaled table.
 template <typename T, int W, int D, int P>
 struct ScaledRow {
+    static constexpr bool kFold = false;
     const T* row;
     const int (&u)[P];
     template <int d>
@@ -121,6 +122,7 @@ struct ScaledRow {
 // x_a / d computed from the raw increment held in registers.
 template <typename T, int W, int D, int P>
 struct RawRow {
+    static constexpr bool kFold = false;
     T raw[W];
     T rawu[P];
     RPY_HD inline __attribute__((always_inline)) RawRow(const T* row, const int (&u)[P])
@@ -140,6 +142,27 @@ struct RawRow {
     RPY_HD inline __attribute__((always_inline)) T x(int a) const { return sc<d>(raw[a]); }
     template <int d>
     RPY_HD inline __attribute__((always_inline)) T xu(int j) const { return sc<d>(rawu[j]); }
+};
+
+// The row of one step as the stream-per-workgroup kernel holds it: the raw
+// increment x_a (the same for every lane of a stream) in registers, and the
+// lane's path letters x_{u_j} / d from a per-step table of scaled values,
+// letter-major with `LS` slots per letter (tab[a * LS + d - 1] = x_a / d).
+// A subtree stage whose divisor d > 1 scales its partial by 1/d once
+// (kFold) instead of reading scaled subtree values.
+template <typename T, int W, int D, int P, int LS>
+struct UniformRow {
+    static constexpr bool kFold = true;
+    const T (&raw)[W];
+    const T* const (&tab)[P];  // tab[j] = the table row of letter u_j
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T x(int a) const
+    {
+        static_assert(d == 1, "folded rows hold the raw increment only");
+        return raw[a];
+    }
+    template <int d>
+    RPY_HD inline __attribute__((always_inline)) T xu(int j) const { return tab[j][d - 1]; }
 };
 
 template <typename T, int W, int D, int P>
@@ -228,10 +251,19 @@ struct Lane {
     RPY_HD inline __attribute__((always_inline)) void stage(T qprev, const Acc& acc)
     {
         constexpr int O = S::off(K);
+        constexpr int d = M - K + 1;
+        // folded rows: q_{K-1} / d once, times the raw x_a for each child
+        // (exact for d a power of two)
+        const T qs = (Acc::kFold && d > 1) ? qprev * (T(1) / T(d)) : qprev;
         static_for<W>([&](auto a) {
             constexpr int ai = int(decltype(a)::value);
             constexpr int v = V * W + ai;
-            const T q = fmaT(qprev, acc.template x<M - K + 1>(ai), sub[O + v]);
+            T xa;
+            if constexpr (Acc::kFold)
+                xa = acc.template x<1>(ai);
+            else
+                xa = acc.template x<d>(ai);
+            const T q = fmaT(qs, xa, sub[O + v]);
             if constexpr (K == M)
                 sub[O + v] = q;
             else
