@@ -78,9 +78,9 @@ def _bounds_of(interval):
 
 
 class Partition(RealInterval):
-    """A clopen interval [inf, sup) cut at sorted intermediate points, mirror
-    of intervals::Partition (intervals/src/partition.cpp:44-250) and its
-    Python binding (roughpy/src/intervals/partition.cpp:51-130).
+    """A clopen interval [inf, sup) cut at sorted intermediate points, with
+    the behaviour of intervals::Partition (intervals/src/partition.cpp:44-250)
+    and its Python binding (roughpy/src/intervals/partition.cpp:51-130).
 
     Intermediates outside [inf, sup) or equal to an end point are dropped;
     ``p[i]`` is the i-th sub-interval, left to right."""
@@ -90,11 +90,8 @@ class Partition(RealInterval):
     def __init__(self, interval, intermediates=()):
         inf_, sup_ = _bounds_of(interval)
         super().__init__(inf_, sup_)
-        mids = [float(m) for m in intermediates]
-        self._mids = sorted(m for m in mids if self._contains(m) and m != inf_ and m != sup_)
-
-    def _contains(self, p: float) -> bool:
-        return self.inf <= p < self.sup
+        kept = (float(m) for m in intermediates)
+        self._mids = sorted(m for m in kept if inf_ < m < sup_)
 
     @classmethod
     def _raw(cls, inf_, sup_, mids):
@@ -103,104 +100,72 @@ class Partition(RealInterval):
         obj._mids = list(mids)
         return obj
 
+    def _points(self):
+        """inf, the intermediates and sup, in order."""
+        return [self.inf, *self._mids, self.sup]
+
     def __len__(self):
         return 1 + len(self._mids)
 
     def __getitem__(self, i: int) -> RealInterval:
-        n = len(self._mids)
-        if i < 0 or i > n:
+        if not 0 <= i <= len(self._mids):
             raise IndexError("partition index out of range")
-        if n == 0:
-            return RealInterval(self.inf, self.sup)
-        lo = self.inf if i == 0 else self._mids[i - 1]
-        hi = self.sup if i == n else self._mids[i]
-        return RealInterval(lo, hi)
+        pts = self._points()
+        return RealInterval(pts[i], pts[i + 1])
 
     def intermediates(self):
         return list(self._mids)
 
     def refine_midpoints(self) -> "Partition":
-        """Inserts the midpoint of every sub-interval."""
-        out, prev = [], self.inf
-        for m in self._mids:
-            out.append((m + prev) / 2.0)
-            out.append(m)
-            prev = m
-        out.append((self.sup + prev) / 2.0)
-        return Partition._raw(self.inf, self.sup, out)
+        """Every sub-interval [a, b) cut at (b + a) / 2."""
+        pts = self._points()
+        mids = []
+        for a, b in zip(pts[:-1], pts[1:]):
+            mids.append((b + a) / 2.0)
+            mids.append(b)
+        return Partition._raw(self.inf, self.sup, mids[:-1])
 
     def mesh(self) -> float:
-        """The smallest sub-interval length (what the reference computes,
-        intervals/src/partition.cpp:75-91)."""
-        prev, diff = self.inf, math.inf
-        for m in self._mids:
-            diff = min(diff, abs(m - prev))
-            prev = m
-        return min(diff, abs(self.sup - prev))
+        """The length of the shortest sub-interval (what partition.cpp:75-91
+        computes)."""
+        pts = self._points()
+        return min(abs(b - a) for a, b in zip(pts[:-1], pts[1:]))
 
     def insert_intermediate(self, point: float) -> None:
         point = float(point)
-        if not self._contains(point):
+        if not self.inf <= point < self.sup:
             raise ValueError("provided intermediate does not lie within the interval")
         import bisect
         bisect.insort_left(self._mids, point)
 
     def merge(self, other: "Partition") -> "Partition":
-        """Union of two partitions (intervals/src/partition.cpp:135-250)."""
-        linf, lsup, rinf, rsup = self.inf, self.sup, other.inf, other.sup
-        out = []
-        if lsup < rinf:
-            if lsup != linf:
-                out.append(lsup)
-            if rinf != rsup:
-                out.append(rinf)
-        elif rsup < linf:
-            if rinf != rsup:
-                out.append(rsup)
-            if lsup != linf:
-                out.append(linf)
-        else:
-            L, R = self._mids, other._mids
-            i = j = 0
-            while i < len(L) and L[i] < rinf:
-                out.append(L[i])
-                i += 1
-            while j < len(R) and R[j] < linf:
-                out.append(R[j])
-                j += 1
-            max_inf, min_sup = max(linf, rinf), min(lsup, rsup)
-            if linf != rinf:
-                out.append(max_inf)
-                if j < len(R) and R[j] == max_inf:
-                    j += 1
-                if i < len(L) and L[i] == max_inf:
-                    i += 1
-            while i < len(L) and j < len(R):
-                if L[i] == R[j]:
-                    out.append(L[i])
-                    i += 1
-                    j += 1
-                elif L[i] < R[j]:
-                    out.append(L[i])
-                    i += 1
-                else:
-                    out.append(R[j])
-                    j += 1
-            while i < len(L) and L[i] < rsup:
-                out.append(L[i])
-                i += 1
-            while j < len(R) and R[j] < lsup:
-                out.append(R[j])
-                j += 1
-            if lsup != rsup and min_sup != max_inf:
-                out.append(min_sup)
-                if i < len(L) and L[i] == min_sup:
-                    i += 1
-                if j < len(R) and R[j] == min_sup:
-                    j += 1
-            out.extend(L[i:])
-            out.extend(R[j:])
-        return Partition._raw(min(linf, rinf), max(lsup, rsup), out)
+        """The partition of the hull [min inf, max sup) of two partitions.
+
+        Overlapping or touching intervals: the cut points are every
+        intermediate of either partition, the larger inf when the infs
+        differ, and the smaller sup when the sups differ and it is not that
+        inf; a value cut in both (or a cut that is also such an end point)
+        appears as often as it does in the one that cuts it most.  Disjoint
+        intervals: the two end points facing the gap only -- the reference
+        drops both partitions' intermediates then
+        (intervals/src/partition.cpp:146-161)."""
+        lo, hi = min(self.inf, other.inf), max(self.sup, other.sup)
+        if self.sup < other.inf or other.sup < self.inf:
+            left, right = (self, other) if self.sup < other.inf else (other, self)
+            cuts = [v for v, nonempty in ((left.sup, left.sup != left.inf),
+                                          (right.inf, right.inf != right.sup)) if nonempty]
+            return Partition._raw(lo, hi, cuts)
+        from collections import Counter
+        mult = Counter(self._mids)
+        for v, c in Counter(other._mids).items():
+            mult[v] = max(mult[v], c)
+        inner_inf, inner_sup = max(self.inf, other.inf), min(self.sup, other.sup)
+        if self.inf != other.inf:
+            mult[inner_inf] = max(mult[inner_inf], 1)
+        if self.sup != other.sup and inner_sup != inner_inf:
+            mult[inner_sup] = max(mult[inner_sup], 1)
+        cuts = [v for v in sorted(mult) for _ in range(mult[v])]
+        return Partition._raw(lo, hi, cuts)
 
     def __eq__(self, other):
         return (isinstance(other, Partition) and self.inf == other.inf

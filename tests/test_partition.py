@@ -60,3 +60,27 @@ def test_merge():
     assert m2.intermediates() == [1.0, 3.0]
     # merging with itself keeps it
     assert a.merge(a) == a
+
+
+@pytest.mark.parametrize("a,b,want", [
+    # touching intervals: the shared end point is one cut
+    (((0.0, 1.0), []), ((1.0, 2.0), []), (0.0, 2.0, [1.0])),
+    # nested: intermediates of both and the inner interval's ends
+    (((0.0, 3.0), [0.5, 2.5]), ((1.0, 2.0), [1.5]), (0.0, 3.0, [0.5, 1.0, 1.5, 2.0, 2.5])),
+    # equal infs: no inner inf; the smaller sup coincides with an intermediate
+    (((0.0, 2.0), [1.0]), ((0.0, 3.0), [2.0]), (0.0, 3.0, [1.0, 2.0])),
+    # an empty interval at the other's sup still cuts there (partition.cpp:186-196)
+    (((3.0, 3.0), []), ((1.0, 3.0), []), (1.0, 3.0, [3.0])),
+    # a repeated intermediate keeps its multiplicity
+    (((0.0, 1.0), [0.5, 0.5]), ((0.0, 1.0), [0.5]), (0.0, 1.0, [0.5, 0.5])),
+    # disjoint: only the end points facing the gap (intermediates dropped)
+    (((0.0, 1.0), [0.5]), ((3.0, 4.0), [3.5]), (0.0, 4.0, [1.0, 3.0])),
+    (((3.0, 4.0), [3.5]), ((0.0, 1.0), [0.5]), (0.0, 4.0, [1.0, 3.0])),
+])
+def test_merge_cases(a, b, want):
+    """Known answers of intervals::Partition::merge
+    (intervals/src/partition.cpp:135-250), worked through its loop by hand."""
+    p = Partition(RealInterval(*a[0]), a[1])
+    q = Partition(RealInterval(*b[0]), b[1])
+    m = p.merge(q)
+    assert (m.inf, m.sup, m.intermediates()) == want
