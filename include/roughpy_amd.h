@@ -459,8 +459,9 @@ int rpy_debug_vjp_host_emulation(double *grad, const double *increments,
                                  int32_t prefix, const double *signature,
                                  const double *grad_signature);
 
-/* Prefix length the signature kernel uses for (width, depth, dtype), or 0 when
- * the shape runs on the generic kernel. */
+/* Prefix length of the signature kernel variant a call of (width, depth,
+ * RPY_F64 if is_f64 else RPY_F32) runs (rpy_tune_config), or 0 when it runs
+ * the generic kernel. */
 int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth,
                                    int32_t is_f64);
 
@@ -469,6 +470,54 @@ int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth,
 int32_t rpy_debug_dyadic_cover(int64_t begin, int64_t end, int32_t resolution,
                                int64_t *k_out, int32_t *n_out,
                                int32_t max_pieces);
+
+/* ---- kernel variants and the tuned table (host only; no counterpart in
+ * the reference, whose CPU kernels have no launch shape to choose) -------
+ *
+ * The signature forward and backward each have several compiled kernel
+ * variants per (width, depth, dtype).  A call runs, in order of precedence:
+ * the variant chosen with rpy_tune_select in this process; the row of the
+ * tuned table compiled into the library (written by the autotuner,
+ * roughpy_amd/tune.py, from timings on MI355X); else the built-in default.
+ * All variants of one shape compute the same signature (up to rounding). */
+
+typedef enum rpy_kernel_op {
+    RPY_OP_SIGNATURE = 0,      /* rpy_stream_signature_batch and friends */
+    RPY_OP_SIGNATURE_VJP = 1   /* rpy_stream_signature_vjp */
+} rpy_kernel_op;
+
+typedef enum rpy_kernel_kind {
+    RPY_KIND_GENERIC = 0,      /* one workgroup per stream, state in memory */
+    RPY_KIND_LANE_CHUNK = 1,   /* 64 (stream, prefix) lanes per wave */
+    RPY_KIND_STREAM_GROUP = 2, /* W^P lanes of one stream per workgroup */
+    RPY_KIND_WAVE_VJP = 3      /* backward: W^P lanes per stream, registers */
+} rpy_kernel_kind;
+
+typedef struct rpy_kernel_variant {
+    int32_t kind;          /* rpy_kernel_kind */
+    int32_t prefix;        /* prefix length P (W^P lanes per stream); 0 generic */
+    int32_t block;         /* increment rows per LDS block (stream group), else 0 */
+    int32_t waves_per_eu;  /* occupancy hint of the lane-chunk kernel, else 0 */
+} rpy_kernel_variant;
+
+/* Writes up to `max` of the variants compiled for (op, width, depth, dtype)
+ * to `out` (may be NULL with max 0); returns how many there are. */
+int32_t rpy_tune_variants(int32_t op, int32_t width, int32_t depth,
+                          rpy_dtype dtype, rpy_kernel_variant *out,
+                          int32_t max);
+
+/* Makes later calls of (op, width, depth, dtype) in this process run
+ * `variant` (one of rpy_tune_variants'), or restores the tuned / default
+ * choice when `variant` is NULL.  RPY_ERR_UNSUPPORTED for a variant that is
+ * not compiled.  Not synchronised with calls in flight on other threads. */
+int rpy_tune_select(int32_t op, int32_t width, int32_t depth, rpy_dtype dtype,
+                    const rpy_kernel_variant *variant);
+
+/* The variant a call of (op, width, depth, dtype) runs now; *source (may be
+ * NULL) is 0 for the built-in default, 1 for the tuned table, 2 for a
+ * rpy_tune_select choice. */
+int rpy_tune_config(int32_t op, int32_t width, int32_t depth, rpy_dtype dtype,
+                    rpy_kernel_variant *out, int32_t *source);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,14 @@ RPY_F32 = 0
 RPY_F64 = 1
 RPY_F32_COMPENSATED = 2
 
+# rpy_kernel_op / rpy_kernel_kind
+RPY_OP_SIGNATURE = 0
+RPY_OP_SIGNATURE_VJP = 1
+RPY_KIND_GENERIC = 0
+RPY_KIND_LANE_CHUNK = 1
+RPY_KIND_STREAM_GROUP = 2
+RPY_KIND_WAVE_VJP = 3
+
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -40,6 +48,14 @@ class TensorBasisC(ctypes.Structure):
 
 class DegreeBoundsC(ctypes.Structure):
     _fields_ = [("min_degree", _i32), ("max_degree", _i32)]
+
+
+class KernelVariantC(ctypes.Structure):
+    """rpy_kernel_variant: kind, prefix, block, waves_per_eu."""
+    _fields_ = [("kind", _i32), ("prefix", _i32), ("block", _i32), ("waves_per_eu", _i32)]
+
+    def astuple(self):
+        return (self.kind, self.prefix, self.block, self.waves_per_eu)
 
 
 class LieBasisC(ctypes.Structure):
@@ -102,6 +118,12 @@ SIGNATURES = {
     "rpy_debug_signature_host_emulation": (ctypes.c_int, [_p_dbl, _p_dbl, _i64, _i32, _i32,
                                                           _i32]),
     "rpy_debug_signature_prefix": (_i32, [_i32, _i32, _i32]),
+    "rpy_tune_variants": (_i32, [_i32, _i32, _i32, ctypes.c_int, ctypes.POINTER(KernelVariantC),
+                                 _i32]),
+    "rpy_tune_select": (ctypes.c_int, [_i32, _i32, _i32, ctypes.c_int,
+                                       ctypes.POINTER(KernelVariantC)]),
+    "rpy_tune_config": (ctypes.c_int, [_i32, _i32, _i32, ctypes.c_int,
+                                       ctypes.POINTER(KernelVariantC), ctypes.POINTER(_i32)]),
     "rpy_dyadic_cache_elements": (_i64, [_i64, _i32, _i32, _i32]),
     "rpy_dyadic_cache_build": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32,
                                               _i32, ctypes.c_int, _vp]),

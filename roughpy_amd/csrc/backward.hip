@@ -38,6 +38,7 @@
 #include "chen_bwd.hpp"
 #include "kernel_util.hpp"
 #include "rpy_common.hpp"
+#include "tuning.hpp"
 
 namespace rpy_amd {
 
@@ -380,25 +381,85 @@ void launch_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n
                        (T*)grad, (const T*)incr, offsets, (const T*)sig, (const T*)gsig, b);
 }
 
-// Shapes with a 64-lane prefix split (one wave per stream).  For float64 at
-// (8,4), (4,6) and (2,10) the lane state takes all 256 VGPRs (at (4,6) the
-// compiler also keeps part of it in AGPRs), which still beats splitting the
-// stream over several waves that must meet at a barrier every step.
-template <typename T>
-bool try_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n, const void* sig,
-              const void* gsig, const DevBasis& b, hipStream_t st)
+// Register-resident backward variants (kind RPY_KIND_WAVE_VJP): a
+// workgroup of W^P lanes per stream.  The defaults are 64-lane splits (one
+// wave per stream): for float64 at (8,4), (4,6) and (2,10) the lane state
+// takes all 256 VGPRs (at (4,6) the compiler also keeps part of it in
+// AGPRs), which still beats splitting the stream over several waves that
+// must meet at a barrier every step.  (8,4) with P = 3 and (4,6) with P = 4
+// (512 and 256 lanes, smaller states) are compiled for the autotuner.
+using VjpLauncher = void (*)(void* grad, const void* incr, const int64_t* offsets, int64_t n,
+                             const void* sig, const void* gsig, const DevBasis& b,
+                             hipStream_t st);
+
+struct VjpVariant {
+    int32_t width, depth;
+    rpy_dtype dtype;
+    int32_t prefix;
+    bool is_default;
+    VjpLauncher launch;
+};
+
+template <int W, int D, int P, bool Default>
+void add_wave(std::vector<VjpVariant>& v)
 {
-    const int w = b.width, d = b.depth;
-#define RPY_WAVE(W, D, P)                                                  \
-    if (w == W && d == D) {                                                \
-        launch_wave<T, W, D, P>(grad, incr, offsets, n, sig, gsig, b, st); \
-        return true;                                                       \
-    }
-    RPY_WAVE(8, 2, 2) RPY_WAVE(8, 3, 2) RPY_WAVE(8, 4, 2)
-    RPY_WAVE(4, 3, 3) RPY_WAVE(4, 4, 3) RPY_WAVE(4, 5, 3) RPY_WAVE(4, 6, 3)
-    RPY_WAVE(2, 6, 6) RPY_WAVE(2, 7, 6) RPY_WAVE(2, 8, 6) RPY_WAVE(2, 9, 6) RPY_WAVE(2, 10, 6)
-#undef RPY_WAVE
-    return false;
+    v.push_back({W, D, RPY_F32, P, Default, &launch_wave<float, W, D, P>});
+    v.push_back({W, D, RPY_F64, P, Default, &launch_wave<double, W, D, P>});
+}
+
+const std::vector<VjpVariant>& vjp_registry()
+{
+    static const std::vector<VjpVariant> r = [] {
+        std::vector<VjpVariant> v;
+        add_wave<8, 2, 2, true>(v);
+        add_wave<8, 3, 2, true>(v);
+        add_wave<8, 4, 2, true>(v);
+        add_wave<4, 3, 3, true>(v);
+        add_wave<4, 4, 3, true>(v);
+        add_wave<4, 5, 3, true>(v);
+        add_wave<4, 6, 3, true>(v);
+        add_wave<2, 6, 6, true>(v);
+        add_wave<2, 7, 6, true>(v);
+        add_wave<2, 8, 6, true>(v);
+        add_wave<2, 9, 6, true>(v);
+        add_wave<2, 10, 6, true>(v);
+        add_wave<8, 4, 3, false>(v);
+        add_wave<4, 6, 4, false>(v);
+        return v;
+    }();
+    return r;
+}
+
+rpy_kernel_variant wave_variant(int32_t prefix)
+{
+    return rpy_kernel_variant{RPY_KIND_WAVE_VJP, prefix, 0, 0};
+}
+
+const VjpVariant* find_vjp(int32_t w, int32_t d, rpy_dtype dt, const rpy_kernel_variant& k)
+{
+    if (k.kind != RPY_KIND_WAVE_VJP || k.block != 0 || k.waves_per_eu != 0) return nullptr;
+    for (const VjpVariant& x : vjp_registry())
+        if (x.width == w && x.depth == d && x.dtype == dt && x.prefix == k.prefix) return &x;
+    return nullptr;
+}
+
+bool vjp_exists(int32_t w, int32_t d, rpy_dtype dt, const rpy_kernel_variant& k)
+{
+    return same_variant(k, rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0}) ||
+           find_vjp(w, d, dt, k) != nullptr;
+}
+
+// The variant a call runs: nullptr for the generic (LDS / workspace) kernel.
+const VjpVariant* choose_vjp(int32_t w, int32_t d, rpy_dtype dt, int32_t* source)
+{
+    rpy_kernel_variant def{RPY_KIND_GENERIC, 0, 0, 0};
+    for (const VjpVariant& x : vjp_registry())
+        if (x.width == w && x.depth == d && x.dtype == dt && x.is_default)
+            def = wave_variant(x.prefix);
+    const rpy_kernel_variant k = resolve_variant(
+            RPY_OP_SIGNATURE_VJP, w, d, dt, def,
+            [&](const rpy_kernel_variant& v) { return vjp_exists(w, d, dt, v); }, source);
+    return find_vjp(w, d, dt, k);
 }
 
 size_t lds_limit(hipStream_t)
@@ -450,12 +511,8 @@ int stream_signature_vjp(void* grad, const void* increments, const int64_t* offs
         return fail(RPY_ERR_SHAPE, "too many streams for one launch");
     const Pow pw = make_pow(width, depth);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const bool wave = dtype == RPY_F64
-                              ? try_wave<double>(grad, increments, offsets, n_streams, signatures,
-                                                 grad_signatures, b, st)
-                              : try_wave<float>(grad, increments, offsets, n_streams, signatures,
-                                                grad_signatures, b, st);
-    if (wave) {
+    if (const VjpVariant* v = choose_vjp(width, depth, dtype, nullptr)) {
+        v->launch(grad, increments, offsets, n_streams, signatures, grad_signatures, b, st);
         RPY_HIP_CHECK(hipGetLastError());
         return RPY_OK;
     }
@@ -535,6 +592,42 @@ int dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_d
         hipLaunchKernelGGL(adj_rmul_kernel<float>, dim3(grid), dim3(kBlock), 0, st, (float*)out,
                            (const float*)op, (const float*)arg, b, batch, os, ps, as);
     RPY_HIP_CHECK(hipGetLastError());
+    return RPY_OK;
+}
+
+int32_t vjp_variants(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+                     int32_t max)
+{
+    if (dtype != RPY_F32 && dtype != RPY_F64) return 0;
+    int32_t n = 0;
+    for (const VjpVariant& x : vjp_registry()) {
+        if (x.width != width || x.depth != depth || x.dtype != dtype) continue;
+        if (out && n < max) out[n] = wave_variant(x.prefix);
+        ++n;
+    }
+    if (out && n < max) out[n] = rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0};
+    return n + 1;
+}
+
+int vjp_select(int32_t width, int32_t depth, rpy_dtype dtype, const rpy_kernel_variant* v)
+{
+    if (dtype != RPY_F32 && dtype != RPY_F64) return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    if (v && !vjp_exists(width, depth, dtype, *v))
+        return fail(RPY_ERR_UNSUPPORTED,
+                    "no signature backward variant (kind %d, prefix %d, block %d, waves/EU %d) "
+                    "for width %d depth %d",
+                    v->kind, v->prefix, v->block, v->waves_per_eu, width, depth);
+    set_selected_variant(RPY_OP_SIGNATURE_VJP, width, depth, dtype, v);
+    return RPY_OK;
+}
+
+int vjp_config(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+               int32_t* source)
+{
+    if (!out) return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
+    if (dtype != RPY_F32 && dtype != RPY_F64) return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    const VjpVariant* v = choose_vjp(width, depth, dtype, source);
+    *out = v ? wave_variant(v->prefix) : rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0};
     return RPY_OK;
 }
 

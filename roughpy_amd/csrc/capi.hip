@@ -401,10 +401,39 @@ int rpy_debug_vjp_host_emulation(double* grad, const double* increments, int64_t
                               grad_signature);
 }
 
-// Prefix length the specialised signature kernel uses, 0 = generic kernel.
+// Prefix length of the signature kernel variant a call runs, 0 = generic.
 int32_t rpy_debug_signature_prefix(int32_t width, int32_t depth, int32_t is_f64)
 {
     return chen_prefix_length(width, depth, is_f64);
+}
+
+int32_t rpy_tune_variants(int32_t op, int32_t width, int32_t depth, rpy_dtype dtype,
+                          rpy_kernel_variant* out, int32_t max)
+{
+    if (max < 0 || (!out && max > 0)) {
+        fail(RPY_ERR_INVALID_ARGUMENT, "bad output array");
+        return 0;
+    }
+    if (op == RPY_OP_SIGNATURE) return sig_variants(width, depth, dtype, out, max);
+    if (op == RPY_OP_SIGNATURE_VJP) return vjp_variants(width, depth, dtype, out, max);
+    fail(RPY_ERR_INVALID_ARGUMENT, "unknown kernel op %d", op);
+    return 0;
+}
+
+int rpy_tune_select(int32_t op, int32_t width, int32_t depth, rpy_dtype dtype,
+                    const rpy_kernel_variant* variant)
+{
+    if (op == RPY_OP_SIGNATURE) return sig_select(width, depth, dtype, variant);
+    if (op == RPY_OP_SIGNATURE_VJP) return vjp_select(width, depth, dtype, variant);
+    return fail(RPY_ERR_INVALID_ARGUMENT, "unknown kernel op %d", op);
+}
+
+int rpy_tune_config(int32_t op, int32_t width, int32_t depth, rpy_dtype dtype,
+                    rpy_kernel_variant* out, int32_t* source)
+{
+    if (op == RPY_OP_SIGNATURE) return sig_config(width, depth, dtype, out, source);
+    if (op == RPY_OP_SIGNATURE_VJP) return vjp_config(width, depth, dtype, out, source);
+    return fail(RPY_ERR_INVALID_ARGUMENT, "unknown kernel op %d", op);
 }
 
 }  // extern "C"

@@ -32,9 +32,41 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
                             const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
                             rpy_dtype dtype, void* stream);
 
-// Prefix length P the specialised kernel uses for (width, depth, dtype), or
-// 0 when the shape runs on the generic kernel.
+// Prefix length P of the variant a call of (width, depth, dtype) runs, or 0
+// when the shape runs on the generic kernel.
 int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64);
+
+// Launches one specialised signature kernel (chen_kernels.hpp) on a sorted
+// stream order; `counter` is the queue counter, cleared by the caller.
+using SigLauncher = int (*)(void* out, const void* incr, const int64_t* begins,
+                            const int64_t* ends, const int32_t* order, int64_t n_streams,
+                            const DevBasis& b, uint32_t* counter, hipStream_t st);
+
+// One compiled signature kernel variant of one shape and dtype.
+struct SigVariant {
+    int32_t width, depth;
+    rpy_dtype dtype;
+    rpy_kernel_variant v;
+    SigLauncher launch;
+};
+
+struct SigVariantList {
+    const SigVariant* items;
+    size_t n;
+};
+
+// Extra variants compiled for the tuned shapes (chen_tune_W_D.hip).
+SigVariantList sig_tune_variants_4_6();
+SigVariantList sig_tune_variants_5_5();
+SigVariantList sig_tune_variants_6_5();
+SigVariantList sig_tune_variants_8_4();
+
+// rpy_tune_* of the signature forward (op RPY_OP_SIGNATURE).
+int32_t sig_variants(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+                     int32_t max);
+int sig_select(int32_t width, int32_t depth, rpy_dtype dtype, const rpy_kernel_variant* v);
+int sig_config(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+               int32_t* source);
 
 // Host run of the lane decomposition for one double-precision stream.
 int signature_host_emulation(double* out, const double* incr, int64_t len, int32_t width,
@@ -62,6 +94,12 @@ int vjp_host_emulation(double* grad, const double* incr, int64_t len, int32_t wi
 int stream_signature_vjp(void* grad, const void* increments, const int64_t* offsets,
                          int64_t n_streams, int32_t width, int32_t depth, const void* signatures,
                          const void* grad_signatures, rpy_dtype dtype, void* stream);
+// rpy_tune_* of the signature backward (op RPY_OP_SIGNATURE_VJP).
+int32_t vjp_variants(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+                     int32_t max);
+int vjp_select(int32_t width, int32_t depth, rpy_dtype dtype, const rpy_kernel_variant* v);
+int vjp_config(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
+               int32_t* source);
 int dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_dtype dtype,
                                const rpy_tensor_basis* basis, int64_t batch, int64_t os,
                                int64_t ps, int64_t as, void* stream);

@@ -199,8 +199,8 @@ def test_stream_row_selection_matches_oracle(rng):
 
 
 @pytest.mark.parametrize("w,d,p", [(2, 3, 1), (2, 3, 3), (3, 4, 1), (3, 4, 2), (5, 5, 2),
-                                   (5, 5, 3), (8, 4, 2), (8, 4, 3), (6, 5, 3), (4, 6, 3),
-                                   (4, 6, 4)])
+                                   (5, 5, 3), (5, 5, 4), (8, 4, 2), (8, 4, 3), (6, 5, 3),
+                                   (6, 5, 4), (4, 6, 3), (4, 6, 4), (4, 6, 5)])
 def test_kernel_lane_decomposition_on_host(rng, w, d, p):
     """The signature kernel's per-lane algorithm and output index map, run on
     the host, equals the oracle's Chen signature."""
@@ -213,13 +213,81 @@ def test_kernel_lane_decomposition_on_host(rng, w, d, p):
     assert O.level_errors(out, O.signature(inc, w, d), w, d).max() < 1e-13
 
 
+def _tuned_rows():
+    """Rows of the compiled tuned table, parsed from its source."""
+    path = os.path.join(ROOT, "roughpy_amd", "csrc", "signature_tuned.inc")
+    ops = {"RPY_OP_SIGNATURE": 0, "RPY_OP_SIGNATURE_VJP": 1}
+    dts = {"RPY_F32": 0, "RPY_F64": 1, "RPY_F32_COMPENSATED": 2}
+    pat = re.compile(r"^\s*\{(RPY_OP_\w+), (\d+), (\d+), (RPY_F\w+), "
+                     r"\{(\d+), (\d+), (\d+), (\d+)\}\},")
+    rows = []
+    for line in open(path):
+        m = pat.match(line)
+        if m:
+            rows.append((ops[m.group(1)], int(m.group(2)), int(m.group(3)), dts[m.group(4)],
+                         tuple(int(m.group(i)) for i in range(5, 9))))
+    return rows
+
+
+def _config(L, op, w, d, dt):
+    c = _capi.KernelVariantC()
+    src = ctypes.c_int32()
+    _capi.check(L.rpy_tune_config(op, w, d, dt, ctypes.byref(c), ctypes.byref(src)))
+    return c.astuple(), src.value
+
+
+def _variants(L, op, w, d, dt):
+    n = L.rpy_tune_variants(op, w, d, dt, None, 0)
+    arr = (_capi.KernelVariantC * n)()
+    assert L.rpy_tune_variants(op, w, d, dt, arr, n) == n
+    return [a.astuple() for a in arr]
+
+
 def test_kernel_prefix_choice():
+    """rpy_debug_signature_prefix reports the prefix of the variant that runs;
+    without a tuned row that is the register-budget prefix."""
     L = _capi.lib()
-    assert L.rpy_debug_signature_prefix(5, 5, 0) == 2
-    assert L.rpy_debug_signature_prefix(5, 5, 1) == 3
-    assert L.rpy_debug_signature_prefix(8, 4, 1) == 2      # 75 float64 values per lane
-    assert L.rpy_debug_signature_prefix(4, 6, 1) == 4      # P = 3 would hold 88
+    tuned = {(w, d, dt) for op, w, d, dt, _ in _tuned_rows() if op == 0}
+    for w, d, f64, p in [(5, 5, 0, 2), (5, 5, 1, 3), (8, 4, 1, 2), (4, 6, 1, 4), (3, 4, 0, 1)]:
+        got = L.rpy_debug_signature_prefix(w, d, f64)
+        assert got == _config(L, 0, w, d, f64)[0][1]
+        if (w, d, f64) not in tuned:
+            assert got == p
     assert L.rpy_debug_signature_prefix(9, 3, 1) == 0      # generic kernel
+
+
+def test_tuned_table_is_what_dispatch_runs():
+    """Every row of the tuned table names a compiled variant and is the
+    configuration a call of that shape runs (source 1 = tuned table)."""
+    L = _capi.lib()
+    for op, w, d, dt, v in _tuned_rows():
+        assert v in _variants(L, op, w, d, dt), (op, w, d, dt, v)
+        assert _config(L, op, w, d, dt) == (v, 1), (op, w, d, dt)
+
+
+def test_kernel_variant_selection():
+    """rpy_tune_select forces a compiled variant and NULL restores the tuned
+    or default choice; variants that are not compiled are refused."""
+    L = _capi.lib()
+    base = _config(L, 0, 5, 5, 1)
+    vs = _variants(L, 0, 5, 5, 1)
+    assert vs[-1] == (0, 0, 0, 0)                    # the generic kernel runs every shape
+    assert len(vs) >= 5 and len(set(vs)) == len(vs)
+    for v in vs:
+        _capi.check(L.rpy_tune_select(0, 5, 5, 1, ctypes.byref(_capi.KernelVariantC(*v))))
+        assert _config(L, 0, 5, 5, 1) == (v, 2)
+        assert L.rpy_debug_signature_prefix(5, 5, 1) == v[1]
+    _capi.check(L.rpy_tune_select(0, 5, 5, 1, None))
+    assert _config(L, 0, 5, 5, 1) == base
+    bad = _capi.KernelVariantC(2, 3, 48, 0)        # no 48-row stream kernel is compiled
+    assert L.rpy_tune_select(0, 5, 5, 1, ctypes.byref(bad)) == _capi.RPY_ERR_UNSUPPORTED
+    assert L.rpy_tune_select(1, 5, 5, 1, ctypes.byref(_capi.KernelVariantC(3, 3, 0, 0))) == \
+        _capi.RPY_ERR_UNSUPPORTED                    # no register backward at width 5
+    assert L.rpy_tune_select(7, 5, 5, 1, None) == _capi.RPY_ERR_INVALID_ARGUMENT
+    assert _variants(L, 0, 9, 3, 0) == [(0, 0, 0, 0)]
+    vjp = _variants(L, 1, 8, 4, 0)
+    assert (3, 2, 0, 0) in vjp and (3, 3, 0, 0) in vjp and vjp[-1] == (0, 0, 0, 0)
+    assert _variants(L, 1, 5, 5, 2) == []            # the backward has no compensated mode
 
 
 @pytest.mark.parametrize("w,d,p", [(8, 2, 2), (8, 3, 2), (8, 4, 2), (8, 4, 3), (4, 3, 3),
