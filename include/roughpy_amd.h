@@ -259,6 +259,18 @@ int rpy_dense_ft_log(void *out, const void *arg, rpy_dtype dtype,
                      int64_t batch, int64_t out_stride, int64_t arg_stride,
                      int64_t out_inc, int64_t arg_inc, void *stream);
 
+/* Cotangent of the truncated log (the log of rpy_dense_ft_log at full
+ * depth; arg's degree 0 is not read): out[e] = d<cotangent[e], log(arg[e])>
+ * / d arg[e], degree 0 of out set to 0.  No counterpart in the reference,
+ * which has no derivatives; it is the log step of the log-signature
+ * backward (Horner chain of the log run in reverse, float64 arithmetic for
+ * either dtype).  Batch strides in elements; each element's last axis is
+ * contiguous. */
+int rpy_dense_ft_log_vjp(void *out, const void *arg, const void *cotangent,
+                         rpy_dtype dtype, const rpy_tensor_basis *basis,
+                         int64_t batch, int64_t out_stride, int64_t arg_stride,
+                         int64_t cotangent_stride, void *stream);
+
 /* out[e] += scale * M arg[e] for a device CSR/CSC matrix; replaces
  * dense_lie_to_tensor / dense_tensor_to_lie (dense_basic.cpp:862-1047,
  * 1116-1298, kernel apply_sparse_linear_map.hpp:16-55).  out has

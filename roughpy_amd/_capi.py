@@ -99,6 +99,9 @@ SIGNATURES = {
                                           _i64, _i64, _i64, _i64, _i64, _i64, _vp]),
     "rpy_dense_ft_log": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _TB, _DB, _DB, _i64, _i64, _i64,
                                         _i64, _i64, _vp]),
+    "rpy_dense_ft_log_vjp": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int,
+                                            ctypes.POINTER(TensorBasisC), _i64, _i64, _i64, _i64,
+                                            _vp]),
     "rpy_dense_sparse_map": (ctypes.c_int, [_vp, _vp, _SM, ctypes.c_int, _dbl, _i64, _i64,
                                             _i64, _i64, _i64, _vp]),
     "rpy_stream_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,

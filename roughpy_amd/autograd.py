@@ -8,9 +8,10 @@
   grad_a = R*_b(g), grad_b = L*_a(g) (adjoints of right / left
   multiplication; the left one is free_tensor_adjoint_left_mul.hpp:10-77).
 * ``log_signature(...)`` -- t2l(log(S)) (lite_context.h:899-902), the log
-  taken in float64 by the Horner series of free_tensor_log.hpp:12-40 built
-  from ``ft_mul``, the t2l projection a sparse map whose backward applies
-  the transposed matrix.
+  taken in float64 by rpy_dense_ft_log (free_tensor_log.hpp:12-40) and the
+  t2l projection a sparse map.  Its backward is hand-written: t2l^T (the
+  transposed sparse map), the reverse Horner chain of the log
+  (rpy_dense_ft_log_vjp), then reverse Chen (rpy_stream_signature_vjp).
 
 All arithmetic runs in the library's HIP kernels; torch only carries device
 memory, streams and the autograd graph.
@@ -116,27 +117,88 @@ def _make_mul_fn():
     return _FtMul
 
 
+def _apply_map(mat: SparseMatrix, x):
+    """y = M x per row of the device tensor x (float64 or float32)."""
+    torch = _torch()
+    out = torch.zeros((x.shape[0], mat.rows), dtype=x.dtype, device=x.device)
+    _capi.check(_capi.lib().rpy_dense_sparse_map(
+        out.data_ptr(), x.data_ptr(), mat.device_struct(), _code(x), 1.0, x.shape[0],
+        mat.rows, x.shape[1], 1, 1, _device.stream_handle()))
+    return out
+
+
 def _make_map_fn():
     torch = _torch()
-
-    def apply(mat: SparseMatrix, x):
-        out = torch.zeros((x.shape[0], mat.rows), dtype=x.dtype, device=x.device)
-        _capi.check(_capi.lib().rpy_dense_sparse_map(
-            out.data_ptr(), x.data_ptr(), mat.device_struct(), _code(x), 1.0, x.shape[0],
-            mat.rows, x.shape[1], 1, 1, _device.stream_handle()))
-        return out
 
     class _Map(torch.autograd.Function):
         @staticmethod
         def forward(ctx, x, mat, mat_t):
             ctx.mat_t = mat_t
-            return apply(mat, x.contiguous())
+            return _apply_map(mat, x.contiguous())
 
         @staticmethod
         def backward(ctx, g):
-            return apply(ctx.mat_t, g.contiguous()), None, None
+            return _apply_map(ctx.mat_t, g.contiguous()), None, None
 
     return _Map
+
+
+def _make_logsig_fn():
+    torch = _torch()
+
+    class _LogSignature(torch.autograd.Function):
+        """t2l(log(S(x))) with the hand-written backward: the cotangent g of
+        the Lie coefficients goes back through t2l^T (sparse map), then the
+        reverse Horner chain of the log (rpy_dense_ft_log_vjp), then reverse
+        Chen (rpy_stream_signature_vjp); float64 for the map and the log."""
+
+        @staticmethod
+        def forward(ctx, increments, offsets, width, depth, lie_basis):
+            inc = increments.contiguous()
+            offs = offsets.to(torch.int64).contiguous()
+            n = int(offs.shape[0]) - 1
+            tb = TensorBasis(width, depth)
+            size = tb.size()
+            L = _capi.lib()
+            S = torch.empty((n, size), dtype=inc.dtype, device=inc.device)
+            _capi.check(L.rpy_stream_signature_batch(
+                S.data_ptr(), inc.data_ptr(), offs.data_ptr(), n, width, depth, _code(inc),
+                _device.stream_handle()))
+            S64 = S if S.dtype == torch.float64 else S.double()
+            logS = torch.empty((n, size), dtype=torch.float64, device=inc.device)
+            B = _capi.DegreeBoundsC
+            _capi.check(L.rpy_dense_ft_log(
+                logS.data_ptr(), S64.data_ptr(), _capi.RPY_F64, tb.c_struct(), B(0, -1), B(0, -1),
+                n, size, size, 1, 1, _device.stream_handle()))
+            t2l = lie_basis.get_t2l_matrix(np.float64)
+            out = _apply_map(t2l, logS)
+            ctx.save_for_backward(inc, offs, S)
+            ctx.args = (width, depth, lie_basis, tb)
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            inc, offs, S = ctx.saved_tensors
+            width, depth, lie_basis, tb = ctx.args
+            n, size = S.shape
+            L = _capi.lib()
+            t2l = lie_basis.get_t2l_matrix(np.float64)
+            if getattr(t2l, "_transposed", None) is None:
+                t2l._transposed = _transpose(t2l)
+            H = _apply_map(t2l._transposed, g.to(torch.float64).contiguous())
+            S64 = S if S.dtype == torch.float64 else S.double()
+            hS = torch.empty((n, size), dtype=torch.float64, device=S.device)
+            _capi.check(L.rpy_dense_ft_log_vjp(
+                hS.data_ptr(), S64.data_ptr(), H.data_ptr(), _capi.RPY_F64, tb.c_struct(), n,
+                size, size, size, _device.stream_handle()))
+            hS = hS.to(inc.dtype).contiguous()
+            grad = torch.zeros_like(inc)
+            _capi.check(L.rpy_stream_signature_vjp(
+                grad.data_ptr(), inc.data_ptr(), offs.data_ptr(), n, width, depth, S.data_ptr(),
+                hS.data_ptr(), _code(inc), _device.stream_handle()))
+            return grad, None, None, None, None
+
+    return _LogSignature
 
 
 _FNS = {}
@@ -144,7 +206,8 @@ _FNS = {}
 
 def _fn(name):
     if name not in _FNS:
-        _FNS[name] = {"sig": _make_signature_fn, "mul": _make_mul_fn, "map": _make_map_fn}[name]()
+        _FNS[name] = {"sig": _make_signature_fn, "mul": _make_mul_fn, "map": _make_map_fn,
+                      "logsig": _make_logsig_fn}[name]()
     return _FNS[name]
 
 
@@ -180,22 +243,11 @@ def sparse_map(mat: SparseMatrix, x, mat_t: SparseMatrix = None):
 
 
 def log_signature(increments, offsets, width: int, depth: int, lie_basis: LieBasis = None):
-    """(n, lie_size) float64 log-signatures in the Hall basis, differentiable."""
+    """(n, lie_size) float64 log-signatures in the Hall basis, differentiable
+    w.r.t. the increments through the hand-written backward."""
     torch = _torch()
     lie_basis = lie_basis or LieBasis(width, depth)
-    tb = TensorBasis(width, depth)
-    S = signature(increments, offsets, width, depth).to(torch.float64)
-    # Y = S - 1 (the unit coefficient is 1 for a signature)
-    mask = torch.ones(tb.size(), dtype=torch.float64, device=S.device)
-    mask[0] = 0.0
-    Y = S * mask
-    unit = torch.zeros_like(S)
-    unit[:, 0] = 1.0
-    out = torch.zeros_like(S)
-    for deg in range(depth, 0, -1):
-        out = out + unit * ((1.0 if deg % 2 else -1.0) / deg)
-        out = ft_mul(out, Y, basis=tb)
-    t2l = lie_basis.get_t2l_matrix(np.float64)
-    if getattr(t2l, "_transposed", None) is None:
-        t2l._transposed = _transpose(t2l)
-    return sparse_map(t2l, out, t2l._transposed)
+    inc = _device.to_device(increments) if not isinstance(increments, torch.Tensor) else increments
+    offs = _device.to_device(np.asarray(offsets, dtype=np.int64)) if not isinstance(
+        offsets, torch.Tensor) else offsets.to(inc.device)
+    return _fn("logsig").apply(inc, offs, int(width), int(depth), lie_basis)

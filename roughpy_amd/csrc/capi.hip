@@ -298,6 +298,14 @@ int rpy_dense_ft_log(void* out, const void* arg, rpy_dtype dtype, const rpy_tens
     });
 }
 
+int rpy_dense_ft_log_vjp(void* out, const void* arg, const void* cotangent, rpy_dtype dtype,
+                         const rpy_tensor_basis* basis, int64_t batch, int64_t out_stride,
+                         int64_t arg_stride, int64_t cotangent_stride, void* stream)
+{
+    return dense_ft_log_vjp(out, arg, cotangent, dtype, basis, batch, out_stride, arg_stride,
+                            cotangent_stride, stream);
+}
+
 int rpy_dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* matrix,
                          rpy_dtype dtype, double scale, int64_t batch, int64_t os, int64_t as,
                          int64_t oi, int64_t ai, void* stream)

@@ -104,6 +104,11 @@ int dense_ft_adjoint_right_mul(void* out, const void* op, const void* arg, rpy_d
                                const rpy_tensor_basis* basis, int64_t batch, int64_t os,
                                int64_t ps, int64_t as, void* stream);
 
+// log_vjp.hip
+int dense_ft_log_vjp(void* out, const void* arg, const void* cotangent, rpy_dtype dtype,
+                     const rpy_tensor_basis* basis, int64_t batch, int64_t os, int64_t as,
+                     int64_t cs, void* stream);
+
 // gram.hip
 int gram_bf16x3(float* G, const float* S, int64_t n, int64_t k, int64_t lds, int64_t ldg,
                 void* stream);

@@ -17,8 +17,10 @@ import pytest
 from numpy.testing import assert_allclose
 
 from oracle import oracle as O
+from roughpy_amd import _capi, _device
 from roughpy_amd import autograd as ag
 from roughpy_amd import streams as rs
+from roughpy_amd.bases import TensorBasis
 
 pytestmark = pytest.mark.gpu
 
@@ -261,3 +263,83 @@ def test_log_signature_vjp_w8_d4_4096_streams(gpu):
     print(f"log-signature w8 d4 4096x256: float64 gradient vs central differences, "
           f"worst {worst:.2e}")
     assert worst < 1e-6
+
+
+@pytest.mark.parametrize("w,d,batch", [(3, 4, 6), (8, 4, 5), (5, 5, 3), (4, 6, 3), (2, 7, 4)])
+def test_ft_log_vjp_against_central_differences(gpu, w, d, batch):
+    """rpy_dense_ft_log_vjp (reverse Horner chain of the log) against float64
+    central differences (step 1e-6) of <H, log(S)> with the oracle's log, on
+    sampled coefficients of every degree; (4,6) and (2,7) take the
+    global-workspace path (the four float64 tensors exceed LDS).  The float32
+    call equals the float64 one to float32 rounding."""
+    torch = gpu
+    rng = np.random.default_rng(300 + 10 * w + d)
+    tb = TensorBasis(w, d)
+    n = tb.size()
+    db = O.tensor_degree_begin(w, d)
+    S = np.zeros((batch, n))
+    S[:, 0] = 1.0
+    for k in range(1, d + 1):
+        S[:, db[k]:db[k + 1]] = rng.uniform(-1, 1, size=(batch, db[k + 1] - db[k])) / 2 ** k
+    H = rng.normal(size=(batch, n))
+    Sd = torch.from_numpy(S).cuda()
+    Hd = torch.from_numpy(H).cuda()
+    out = torch.full((batch, n), float("nan"), dtype=torch.float64, device="cuda")
+    L = _capi.lib()
+    _capi.check(L.rpy_dense_ft_log_vjp(out.data_ptr(), Sd.data_ptr(), Hd.data_ptr(),
+                                       _capi.RPY_F64, tb.c_struct(), batch, n, n, n,
+                                       _device.stream_handle()))
+    got = out.cpu().numpy()
+    assert (got[:, 0] == 0).all()
+    h = 1e-6
+    for e in range(batch):
+        idx = [int(rng.integers(db[k], db[k + 1])) for k in range(1, d + 1) for _ in range(3)]
+        for i in idx:
+            sp, sm = S[e].copy(), S[e].copy()
+            sp[i] += h
+            sm[i] -= h
+            fd = (np.dot(H[e], O.ft_log(sp, w, d)) - np.dot(H[e], O.ft_log(sm, w, d))) / (2 * h)
+            assert abs(got[e, i] - fd) <= 1e-6 * max(1.0, abs(fd)), (e, i, got[e, i], fd)
+    out32 = torch.full((batch, n), float("nan"), dtype=torch.float32, device="cuda")
+    S32, H32 = Sd.float(), Hd.float()
+    _capi.check(L.rpy_dense_ft_log_vjp(out32.data_ptr(), S32.data_ptr(), H32.data_ptr(),
+                                       _capi.RPY_F32, tb.c_struct(), batch, n, n, n,
+                                       _device.stream_handle()))
+    S32d, H32d = S32.double(), H32.double()  # bound: the call reads them
+    _capi.check(L.rpy_dense_ft_log_vjp(out.data_ptr(), S32d.data_ptr(), H32d.data_ptr(),
+                                       _capi.RPY_F64, tb.c_struct(), batch, n, n, n,
+                                       _device.stream_handle()))
+    ref32 = out.cpu().numpy()
+    assert_allclose(out32.cpu().numpy(), ref32, rtol=1e-6, atol=1e-6 * np.abs(ref32).max())
+
+
+def test_ft_log_vjp_equals_the_composed_backward(gpu, rng):
+    """The log cotangent kernel equals reverse mode through the Horner chain
+    built from ft_mul autograd nodes (adjoint multiplication kernels), to
+    float64 rounding."""
+    torch = gpu
+    w, d, batch = 4, 4, 7
+    tb = TensorBasis(w, d)
+    n = tb.size()
+    S = rng.uniform(-0.3, 0.3, size=(batch, n))
+    S[:, 0] = 1.0
+    H = rng.normal(size=(batch, n))
+    Sd = torch.from_numpy(S).cuda().requires_grad_(True)
+    mask = torch.ones(n, dtype=torch.float64, device="cuda")
+    mask[0] = 0.0
+    Y = Sd * mask
+    unit = torch.zeros((batch, n), dtype=torch.float64, device="cuda")
+    unit[:, 0] = 1.0
+    acc = torch.zeros_like(unit)
+    for deg in range(d, 0, -1):
+        acc = acc + unit * ((1.0 if deg % 2 else -1.0) / deg)
+        acc = ag.ft_mul(acc, Y, basis=tb)
+    Hd = torch.from_numpy(H).cuda()
+    (acc * Hd).sum().backward()
+    composed = Sd.grad.cpu().numpy()
+    out = torch.empty((batch, n), dtype=torch.float64, device="cuda")
+    Sc = torch.from_numpy(S).cuda()
+    _capi.check(_capi.lib().rpy_dense_ft_log_vjp(out.data_ptr(), Sc.data_ptr(), Hd.data_ptr(),
+                                                 _capi.RPY_F64, tb.c_struct(), batch, n, n, n,
+                                                 _device.stream_handle()))
+    assert_allclose(out.cpu().numpy(), composed, rtol=1e-12, atol=1e-12 * np.abs(composed).max())
