@@ -35,6 +35,11 @@ PEAK_HBM_GBS = 8000.0
 PROFILE = os.path.join(ROOT, "profiles", "chen_signature_w5_d5_16384streams.md")
 
 
+def progress(msg):
+    """One line per finished step on stderr, so a long run shows where it is."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def profiled_traffic(dtype):
     """HBM bytes per launch of the signature kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
@@ -746,6 +751,45 @@ def cpu_baseline(inc, offs, target_seconds=8.0):
     }
 
 
+def compensated_arithmetic():
+    """What the compensated mode of the headline shape runs."""
+    from roughpy_amd import tune
+    from roughpy_amd import _capi
+    v, _ = tune.config(_capi.RPY_OP_SIGNATURE, W, D, _capi.RPY_F32_COMPENSATED)
+    if v[0] == _capi.RPY_KIND_BLOCK_MERGE:
+        return (f"float32 segments of {v[2]} rows (at most 16 per stream) on the compensated "
+                "lanes, merged per stream with Kahan sums")
+    if v[0] in (_capi.RPY_KIND_LANE_CHUNK, _capi.RPY_KIND_STREAM_GROUP):
+        return (f"float32 Chen steps ({tune.describe(v)}), signature levels 1 and 2 "
+                "accumulated with Kahan's compensated summation")
+    return "float64 arithmetic on float32 data, each coefficient rounded once"
+
+
+def block_merge_leg(torch, dinc32, doffs, s64, steps, warmup):
+    """The compensated mode's block-merge variant: segments of T rows on the
+    compensated lanes, merged per stream with Kahan sums."""
+    from roughpy_amd import _capi, _device, tune
+    L = _capi.lib()
+    n = int(doffs.shape[0]) - 1
+    out = torch.empty((n, int(L.rpy_tensor_size(W, D))), dtype=torch.float32, device="cuda")
+    res = {}
+    for T in (256, 512):
+        v = (_capi.RPY_KIND_BLOCK_MERGE, 0, T, 0)
+        tune.select(_capi.RPY_OP_SIGNATURE, W, D, _capi.RPY_F32_COMPENSATED, v)
+        try:
+            def fn():
+                _capi.check(L.rpy_stream_signature_batch(out.data_ptr(), dinc32.data_ptr(),
+                                                         doffs.data_ptr(), n, W, D,
+                                                         _capi.RPY_F32_COMPENSATED,
+                                                         _device.stream_handle()))
+            ms = time_kernel(fn, steps, warmup, torch)
+        finally:
+            tune.select(_capi.RPY_OP_SIGNATURE, W, D, _capi.RPY_F32_COMPENSATED, None)
+        res[f"segments_{T}"] = {"ms_per_step": ms, "value": n / (ms * 1e-3),
+                                "rel_err_vs_f64": level_rel_err(out, s64, W, D)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -807,6 +851,7 @@ def main():
         ms = time_kernel(fn, args.steps if dt == args.dtype else max(2, args.steps // 2),
                          args.warmup, torch)
         results[dt] = (ms, out)
+        progress(f"signatures {dt}: {ms:.3f} ms")
 
     ms, out = results[args.dtype]
     flops = 2.0 * FMA_PER_STEP * total_incr
@@ -853,34 +898,52 @@ def main():
         "f32_compensated_kernel": {
             "ms_per_step": results["f32c"][0],
             "value": N_STREAMS / (results["f32c"][0] * 1e-3),
-            "arithmetic": "float64 on float32 data, rounded once",
+            "speedup_vs_f64": results["f64"][0] / results["f32c"][0],
+            "arithmetic": compensated_arithmetic(),
             "rel_err_vs_f64": level_rel_err(results["f32c"][1], s64, W, D)},
     }
+    line["f32_compensated_kernel"]["block_merge"] = block_merge_leg(
+        torch, dinc32, doffs, s64, max(2, args.steps // 2), 1)
+    line["f32_compensated_kernel"]["block_merge"]["speedup_vs_f64"] = {
+        k: results["f64"][0] / v["ms_per_step"]
+        for k, v in line["f32_compensated_kernel"]["block_merge"].items()}
+    progress("compensated block merge done")
     legs = [x for x in args.legs.split(",") if x]
     if "dyadic" in legs:
         line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
+        progress("dyadic_cache done")
     del dinc64, results, s32, s64, out
     torch.cuda.empty_cache()
     if "fma" in legs:
         line["fma_w4_d6"] = fma_leg(torch, max(3, args.steps // 2), 1)
+        progress("fma_w4_d6 done")
     if "dense" in legs:
         line["dense_w5_d5"] = dense_leg(torch, max(3, args.steps // 2), 1)
+        progress("dense_w5_d5 done")
     if "backward" in legs:
         line["backward_w8_d4"] = backward_leg(torch, max(3, args.steps // 2), 1)
+        progress("backward_w8_d4 done")
     if "logsig" in legs:
         line["log_signature_w8_d4"] = log_signature_leg(torch, max(3, args.steps // 2), 1)
+        progress("log_signature_w8_d4 done")
     if "gram" in legs:
         line["gram_w6_d5"] = gram_leg(torch, max(3, args.steps // 2), 1)
+        progress("gram_w6_d5 done")
     if "shuffle" in legs:
         line["shuffle_w4_d6"] = shuffle_leg(torch, max(3, args.steps // 2), 1)
+        progress("shuffle_w4_d6 done")
     if "simplify" in legs:
         line["simplify_w5_d5"] = simplify_leg(torch, max(3, args.steps // 2))
+        progress("simplify_w5_d5 done")
     if "serialize" in legs:
         line["serialize_cache"] = serialize_leg(torch, inc, offs, lens)
+        progress("serialize_cache done")
     if "graph" in legs:
         line["graph_replay"] = graph_leg(torch)
+        progress("graph_replay done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(inc, offs)
+        progress("cpu_baseline done")
         # the reference CPU kernels beside the secondary legs (one core, float64)
         if "fma_w4_d6" in line:
             line["fma_w4_d6"]["cpu_reference"] = ref_cpu_rates(

@@ -290,9 +290,17 @@ int rpy_dense_sparse_map(void *out, const void *arg,
  * the Chen product of exp(row) over its rows, i.e. Context::compute_signature
  * (algebra/src/lite_context.h:545-563) applied per stream, the value
  * LieIncrementStream.signature() returns for the stream's full support
- * (streams/src/stream.cpp:156-171).  dtype: RPY_F64 (double data),
- * RPY_F32 or RPY_F32_COMPENSATED (float data, float64 arithmetic, each
- * coefficient rounded to float once at the end).  Work is one persistent
+ * (streams/src/stream.cpp:156-171).  dtype: RPY_F64 (double data), RPY_F32
+ * (float data and arithmetic) or RPY_F32_COMPENSATED (float data).  The
+ * compensated mode runs the float32 kernels with the coefficients of levels
+ * 1 and 2 accumulated by Kahan's compensated summation (their rounding
+ * errors are the ones every later step multiplies into the higher levels)
+ * at the shapes with specialised kernels (widths 2..8); its block-merge
+ * variant (RPY_KIND_BLOCK_MERGE, at (3,4), (4,6), (5,5), (6,5) and (8,4))
+ * cuts each stream into at most 16 segments of `block` rows, computes them
+ * on those lanes and multiplies them per stream with Kahan sums; at other
+ * shapes the mode runs float64 arithmetic on the float data, each
+ * coefficient rounded to float once at the end.  Work is one persistent
  * launch over a length-sorted queue. */
 int rpy_stream_signature_batch(void *out, const void *increments,
                                const int64_t *offsets, int64_t n_streams,
@@ -306,7 +314,9 @@ int rpy_stream_signature_batch(void *out, const void *increments,
  * (code under HIP graph capture, tight serving loops, hosts that forbid
  * hipMalloc).  Same arguments, results and errors, plus RPY_ERR_SHAPE when
  * the batch needs more scratch than the array holds (about 300,000 streams;
- * the message states the bytes needed).  The array keeps the last call's
+ * the message states the bytes needed).  The block-compensated mode runs
+ * its segment buffer in rounds of the streams that fit the array (about 30
+ * streams per round at width 5, depth 5).  The array keeps the last call's
  * contents and is shared by every call on that GPU: calls that overlap on
  * one GPU (other streams, other host threads) race and must be ordered by
  * the caller. */
@@ -502,13 +512,17 @@ typedef enum rpy_kernel_kind {
     RPY_KIND_GENERIC = 0,      /* one workgroup per stream, state in memory */
     RPY_KIND_LANE_CHUNK = 1,   /* 64 (stream, prefix) lanes per wave */
     RPY_KIND_STREAM_GROUP = 2, /* W^P lanes of one stream per workgroup */
-    RPY_KIND_WAVE_VJP = 3      /* backward: W^P lanes per stream, registers */
+    RPY_KIND_WAVE_VJP = 3,     /* backward: W^P lanes per stream, registers */
+    RPY_KIND_BLOCK_MERGE = 4   /* RPY_F32_COMPENSATED: segments of `block`
+                                * rows on the compensated lanes, merged per
+                                * stream with Kahan sums */
 } rpy_kernel_kind;
 
 typedef struct rpy_kernel_variant {
     int32_t kind;          /* rpy_kernel_kind */
     int32_t prefix;        /* prefix length P (W^P lanes per stream); 0 generic */
-    int32_t block;         /* increment rows per LDS block (stream group), else 0 */
+    int32_t block;         /* increment rows per LDS block (stream group) or
+                            * per segment (block merge), else 0 */
     int32_t waves_per_eu;  /* occupancy hint of the lane-chunk kernel, else 0 */
 } rpy_kernel_variant;
 

@@ -97,6 +97,12 @@ __global__ __launch_bounds__(256) void chen_generic_kernel(IO* __restrict__ out,
 }
 
 using chen::choose_p;
+
+// Segment lengths of the block-compensated variants, and the default.
+constexpr int32_t kBlockRows[] = {128, 256, 512};
+// The block-compensated segment buffer holds at most this many bytes; larger
+// batches run in rounds of streams.
+constexpr size_t kBlockSegBytes = size_t(16) << 30;
 using chen::kBudgetF32;
 using chen::kBudgetF64;
 
@@ -115,16 +121,18 @@ void add_defaults(std::vector<SigVariant>& v)
                  &launch_persistent<float, W, D, P32>});
     v.push_back({W, D, RPY_F64, {RPY_KIND_LANE_CHUNK, P64, 0, 2},
                  &launch_persistent<double, W, D, P64>});
-    v.push_back({W, D, RPY_F32_COMPENSATED, {RPY_KIND_LANE_CHUNK, P64, 0, 2},
-                 &launch_persistent<double, W, D, P64, float>});
-    if constexpr (stream_ok(W, P32))
+    v.push_back({W, D, RPY_F32_COMPENSATED, {RPY_KIND_LANE_CHUNK, P32, 0, 2},
+                 &launch_persistent<float, W, D, P32, float, 2, true>});
+    if constexpr (stream_ok(W, P32)) {
         v.push_back({W, D, RPY_F32, {RPY_KIND_STREAM_GROUP, P32, 32, 0},
                      &launch_stream<float, W, D, P32>});
+        v.push_back({W, D, RPY_F32_COMPENSATED, {RPY_KIND_STREAM_GROUP, P32, 32, 0},
+                     &launch_stream<float, W, D, P32, float, 32, true>});
+    }
     if constexpr (stream_ok(W, P64)) {
         v.push_back({W, D, RPY_F64, {RPY_KIND_STREAM_GROUP, P64, 32, 0},
                      &launch_stream<double, W, D, P64>});
-        v.push_back({W, D, RPY_F32_COMPENSATED, {RPY_KIND_STREAM_GROUP, P64, 32, 0},
-                     &launch_stream<double, W, D, P64, float>});
+
     }
 }
 
@@ -158,6 +166,14 @@ const std::vector<SigVariant>& registry()
                 const SigVariant& x = list.items[i];
                 if (!find_in(v, x.width, x.depth, x.dtype, x.v)) v.push_back(x);
             }
+        // block-compensated float32 (chen_block.hip): no launcher of its
+        // own, run_signature drives the float32 variant and the merge
+        for (int32_t w = 2; w <= 8; ++w)
+            for (int32_t d = 1; d <= 8; ++d)
+                if (block_merge_supported(w, d))
+                    for (int32_t T : kBlockRows)
+                        v.push_back({w, d, RPY_F32_COMPENSATED,
+                                     {RPY_KIND_BLOCK_MERGE, 0, T, 0}, nullptr});
         return v;
     }();
     return r;
@@ -172,9 +188,13 @@ bool specialised(int32_t w, int32_t d)
 
 rpy_kernel_variant generic_variant() { return rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0}; }
 
+// The register-budget default of a dtype: the stream kernel at prefix
+// choose_p where it exists, else the lane-chunk kernel, else generic.  The
+// compensated mode keeps float32 lanes (with compensated lowest levels), so
+// it takes the float32 budget.
 rpy_kernel_variant default_variant(int32_t w, int32_t d, rpy_dtype dt)
 {
-    const int p = choose_p(w, d, dt == RPY_F32 ? kBudgetF32 : kBudgetF64);
+    const int p = choose_p(w, d, dt == RPY_F64 ? kBudgetF64 : kBudgetF32);
     const rpy_kernel_variant stream{RPY_KIND_STREAM_GROUP, p, 32, 0};
     if (find_in(registry(), w, d, dt, stream)) return stream;
     const rpy_kernel_variant chunk{RPY_KIND_LANE_CHUNK, p, 0, 2};
@@ -263,6 +283,9 @@ struct SigPlan {
     size_t k_bytes, v_bytes;
     size_t total;       // bytes of workspace the call needs
     int generic_grid;
+    // block-compensated float32: the float32 variant run on the segments
+    const SigVariant* seg_var;
+    BlockPlan bp;
 };
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
@@ -272,7 +295,7 @@ size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 int plan_signature(const void* out, const void* increments, const int64_t* begins,
                    const int64_t* ends, int64_t n_streams, int32_t width, int32_t depth,
                    rpy_dtype dtype, hipStream_t st, SigPlan& p, bool* skip,
-                   int max_generic_grid = 4096)
+                   int max_generic_grid = 4096, size_t ws_limit = 0)
 {
     *skip = false;
     if (dtype != RPY_F32 && dtype != RPY_F64 && dtype != RPY_F32_COMPENSATED)
@@ -292,6 +315,27 @@ int plan_signature(const void* out, const void* increments, const int64_t* begin
     p.var = choose_variant(width, depth, dtype, nullptr);
     p.sort_bytes = p.k_bytes = p.v_bytes = 0;
     p.generic_grid = int(std::min<int64_t>(n_streams, max_generic_grid));
+    p.seg_var = nullptr;
+    if (p.var && p.var->v.kind == RPY_KIND_BLOCK_MERGE) {
+        // segments run on the compensated lanes (the default variant)
+        p.seg_var = find_in(registry(), width, depth, RPY_F32_COMPENSATED,
+                            default_variant(width, depth, RPY_F32_COMPENSATED));
+        if (!p.seg_var || !p.seg_var->launch)
+            return fail(RPY_ERR_UNSUPPORTED, "no float32 signature kernel for width %d depth %d",
+                        width, depth);
+        const int64_t per_stream = block_segment_slots() * p.b.size() * int64_t(sizeof(float));
+        int64_t n_sub = std::min<int64_t>(n_streams, std::max<int64_t>(
+                                                  1, int64_t(kBlockSegBytes) / per_stream));
+        n_sub = std::min<int64_t>(n_sub, INT32_MAX / block_segment_slots());
+        if ((rc = block_plan(n_sub, p.b.size(), p.bp, st))) return rc;
+        // a caller with a fixed workspace gets rounds of fewer streams
+        while (ws_limit && p.bp.total > ws_limit && n_sub > 1) {
+            n_sub = (n_sub + 1) / 2;
+            if ((rc = block_plan(n_sub, p.b.size(), p.bp, st))) return rc;
+        }
+        p.total = p.bp.total;
+        return RPY_OK;
+    }
     if (!p.var) {
         p.total = dtype == RPY_F32_COMPENSATED
                           ? align256(size_t(p.generic_grid) * size_t(p.b.size()) * 8)
@@ -314,6 +358,9 @@ int run_signature(void* out, const void* increments, const int64_t* begins, cons
                   hipStream_t st)
 {
     const DevBasis& b = p.b;
+    if (p.var && p.var->v.kind == RPY_KIND_BLOCK_MERGE)
+        return block_run((float*)out, (const float*)increments, begins, ends, n_streams, b,
+                         p.var->v.block, p.seg_var->launch, p.bp, ws, st);
     if (!p.var) {
         const int grid = p.generic_grid;
         if (dtype == RPY_F64)
@@ -362,7 +409,8 @@ int run_signature(void* out, const void* increments, const int64_t* begins, cons
         if (err != hipSuccess)
             return fail(RPY_ERR_HIP, "radix sort failed: %s", hipGetErrorString(err));
     }
-    return p.var->launch(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b, counter, st);
+    return p.var->launch(out, increments, begins, ends, sort ? vals_out : vals_in, n_streams, b,
+                         counter, 0, st);
 }
 
 // Scratch of the entry point that allocates nothing: one array per GPU,
@@ -446,7 +494,7 @@ int stream_signature_batch_noalloc(void* out, const void* increments, const int6
         max_grid = int(std::max<int64_t>(1, std::min<int64_t>(4096, int64_t(kScratchBytes) /
                                                                         (tsize * 8))));
     int rc = plan_signature(out, increments, begins, ends, n_streams, width, depth, dtype, st, p,
-                            &skip, max_grid);
+                            &skip, max_grid, kScratchBytes);
     if (rc || skip) return rc;
     if (p.total > kScratchBytes)
         return fail(RPY_ERR_SHAPE,

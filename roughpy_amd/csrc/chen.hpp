@@ -36,11 +36,17 @@ int stream_signature_ranges(void* out, const void* increments, const int64_t* be
 // when the shape runs on the generic kernel.
 int chen_prefix_length(int32_t width, int32_t depth, int dtype_is_f64);
 
+// Flags of a signature launch.  kSkipEmpty: streams of no rows write
+// nothing (their output rows are left as they are); kSortedDesc: the order
+// is by length, longest first, so the first empty stream ends the queue.
+constexpr int kSkipEmpty = 1;
+constexpr int kSortedDesc = 2;
+
 // Launches one specialised signature kernel (chen_kernels.hpp) on a sorted
 // stream order; `counter` is the queue counter, cleared by the caller.
 using SigLauncher = int (*)(void* out, const void* incr, const int64_t* begins,
                             const int64_t* ends, const int32_t* order, int64_t n_streams,
-                            const DevBasis& b, uint32_t* counter, hipStream_t st);
+                            const DevBasis& b, uint32_t* counter, int flags, hipStream_t st);
 
 // One compiled signature kernel variant of one shape and dtype.
 struct SigVariant {
@@ -67,6 +73,24 @@ int32_t sig_variants(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_v
 int sig_select(int32_t width, int32_t depth, rpy_dtype dtype, const rpy_kernel_variant* v);
 int sig_config(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant* out,
                int32_t* source);
+
+// Block-compensated float32 signatures (chen_block.hip): float32 segment
+// signatures of at most block_segment_slots() segments per stream, merged
+// per stream with compensated float32 sums.
+struct BlockPlan {
+    int64_t n_sub;     // streams per round (the segment buffer holds n_sub streams)
+    int64_t slots;     // n_sub * block_segment_slots()
+    size_t scan_bytes, c_bytes, o_bytes, r_bytes, seg_bytes;
+    size_t total;      // workspace bytes of one call
+};
+bool block_merge_supported(int32_t width, int32_t depth);
+int64_t block_segment_slots();
+int block_plan(int64_t n_sub, int64_t tsize, BlockPlan& p, hipStream_t st);
+// seg_launch: the float32 signature variant run on the segments; T: nominal
+// rows per segment.
+int block_run(float* out, const float* increments, const int64_t* begins, const int64_t* ends,
+              int64_t n_streams, const DevBasis& b, int64_t T, SigLauncher seg_launch,
+              const BlockPlan& p, char* ws, hipStream_t st);
 
 // Host run of the lane decomposition for one double-precision stream.
 int signature_host_emulation(double* out, const double* incr, int64_t len, int32_t width,

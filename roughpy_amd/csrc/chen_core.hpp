@@ -165,17 +165,51 @@ struct UniformRow {
     RPY_HD inline __attribute__((always_inline)) T xu(int j) const { return tab[j][d - 1]; }
 };
 
-template <typename T, int W, int D, int P>
+// Levels whose coefficients the compensated lanes (Comp) accumulate with
+// Kahan's compensated summation.  The rounding errors of the lowest levels
+// are the ones that grow most: every later step multiplies them into all
+// higher levels.  Levels 1 and 2 are one path value and one subtree value
+// of a lane at P = 2, so compensating them costs a few operations per step.
+constexpr int kCompLevels = 2;
+
+template <typename T, int W, int D, int P, bool Comp = false>
 struct Lane {
     using S = Shape<W, D, P>;
+    // compensated path levels 1..CP and subtree positions 0..CS-1 (levels
+    // P..kCompLevels)
+    static constexpr int CP = !Comp ? 0 : (P - 1 < kCompLevels ? P - 1 : kCompLevels);
+    static constexpr int CS = (Comp && kCompLevels >= P) ? S::off(kCompLevels + 1) : 0;
     T path[P];
     T sub[S::SUB];
+    // Kahan terms: the value of a compensated coefficient is hi - c
+    T cpath[CP + 1];
+    T csub[CS > 0 ? CS : 1];
 
     RPY_HD inline __attribute__((always_inline)) void init_identity()
     {
         path[0] = T(1);
         static_for<P - 1>([&](auto k) { path[k + 1] = T(0); });
         static_for<S::SUB>([&](auto i) { sub[i] = T(0); });
+        static_for<CP + 1>([&](auto k) { cpath[k] = T(0); });
+        static_for<(CS > 0 ? CS : 1)>([&](auto i) { csub[i] = T(0); });
+    }
+
+    // Coefficients as written out: subtree position i, path level k.
+    template <int i>
+    RPY_HD inline __attribute__((always_inline)) T sub_out() const
+    {
+        if constexpr (i < CS)
+            return sub[i] - csub[i];
+        else
+            return sub[i];
+    }
+    template <int k>
+    RPY_HD inline __attribute__((always_inline)) T path_out() const
+    {
+        if constexpr (k >= 1 && k <= CP)
+            return path[k] - cpath[k];
+        else
+            return path[k];
     }
 
     // One Chen step.  acc: row accessor (ScaledRow or RawRow) giving x_a / d
@@ -196,6 +230,16 @@ struct Lane {
     }
 
   private:
+
+    // hi += t with Kahan's compensation c (the value is hi - c).  No term
+    // is a product, so contraction cannot change the error term.
+    static RPY_HD inline __attribute__((always_inline)) void kahan(T& hi, T& c, T t)
+    {
+        const T y = t - c;
+        const T s = hi + y;
+        c = (s - hi) - y;
+        hi = s;
+    }
 
     static RPY_HD inline __attribute__((always_inline)) T fmaT(T a, T b, T c)
     {
@@ -227,7 +271,10 @@ struct Lane {
             constexpr int K = int(decltype(k0)::value) + 1;
             p = fmaT(p, acc.template xu<M - K + 1>(K - 1), path[K]);
         });
-        path[M] = fmaT(p, acc.template xu<1>(M - 1), path[M]);
+        if constexpr (M <= CP)
+            kahan(path[M], cpath[M], p * acc.template xu<1>(M - 1));
+        else
+            path[M] = fmaT(p, acc.template xu<1>(M - 1), path[M]);
     }
 
     template <int M, typename Acc>
@@ -235,11 +282,14 @@ struct Lane {
     {
         const T p = path_partial<M>(acc);
         // K = P: the single word u
-        const T q0 = fmaT(p, acc.template xu<M - P + 1>(P - 1), sub[0]);
-        if constexpr (M == P) {
-            sub[0] = q0;
+        if constexpr (M == P && CS > 0) {
+            kahan(sub[0], csub[0], p * acc.template xu<1>(P - 1));
         } else {
-            stage<P + 1, M, 0>(q0, acc);
+            const T q0 = fmaT(p, acc.template xu<M - P + 1>(P - 1), sub[0]);
+            if constexpr (M == P)
+                sub[0] = q0;
+            else
+                stage<P + 1, M, 0>(q0, acc);
         }
     }
 
@@ -263,11 +313,15 @@ struct Lane {
                 xa = acc.template x<1>(ai);
             else
                 xa = acc.template x<d>(ai);
-            const T q = fmaT(qs, xa, sub[O + v]);
-            if constexpr (K == M)
-                sub[O + v] = q;
-            else
-                stage<K + 1, M, v>(q, acc);
+            if constexpr (K == M && O + v < CS) {
+                kahan(sub[O + v], csub[O + v], qs * xa);
+            } else {
+                const T q = fmaT(qs, xa, sub[O + v]);
+                if constexpr (K == M)
+                    sub[O + v] = q;
+                else
+                    stage<K + 1, M, v>(q, acc);
+            }
         });
     }
 };

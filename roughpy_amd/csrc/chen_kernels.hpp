@@ -44,14 +44,15 @@ struct Staging {
     static constexpr int kPerLane = (kVals + kWave - 1) / kWave;
 };
 
-// T: arithmetic type; IO: type of the increments and signatures in memory
-// (float data with double arithmetic is the compensated float32 mode).
-template <typename T, int W, int D, int P, typename IO = T, int WPE = 2>
+// T: arithmetic type; IO: type of the increments and signatures in memory;
+// Comp: Kahan-compensated lowest levels (chen_core.hpp, the compensated
+// float32 mode).
+template <typename T, int W, int D, int P, typename IO = T, int WPE = 2, bool Comp = false>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE)))
 void chen_persistent_kernel(
         IO* __restrict__ out, const IO* __restrict__ incr, const int64_t* __restrict__ begins,
         const int64_t* __restrict__ ends, const int32_t* __restrict__ order, int64_t n_items,
-        int64_t tsize, DevBasis b, uint32_t* __restrict__ counter)
+        int64_t tsize, DevBasis b, uint32_t* __restrict__ counter, int flags)
 {
     using Sh = chen::Shape<W, D, P>;
     constexpr int NP = Sh::NP;
@@ -106,6 +107,11 @@ void chen_persistent_kernel(
         // slot 0)
         int maxlen = 0;
         for (int i = 0; i < n_slots; ++i) maxlen = max(maxlen, int(slot_len[i]));
+        if ((flags & kSkipEmpty) && maxlen == 0) {
+            // longest first: every later chunk is empty as well
+            if (flags & kSortedDesc) break;
+            continue;
+        }
 
         int u[P];
         {
@@ -116,7 +122,7 @@ void chen_persistent_kernel(
             }
         }
 
-        chen::Lane<T, W, D, P> st;
+        chen::Lane<T, W, D, P, Comp> st;
         st.init_identity();
 
         // staging assignment of this lane: values v = lane + 64 i
@@ -166,20 +172,22 @@ void chen_persistent_kernel(
         }
 
         const int s = slot_stream[slot];
-        if (active) {
+        if (active && !((flags & kSkipEmpty) && len == 0)) {
             IO* o = out + int64_t(s) * tsize;
             // subtree levels: lane q owns words q*W^(k-P) .. (q+1)*W^(k-P)-1
             chen::static_for<D - P + 1>([&](auto i) {
                 constexpr int k = P + int(decltype(i)::value);
                 constexpr int nk = int(ipow(W, k - P));
                 IO* ok = o + b.db[k] + int64_t(q) * nk;
-                chen::static_for<nk>([&](auto v) { ok[v] = IO(st.sub[Sh::off(k) + v]); });
+                chen::static_for<nk>([&](auto v) {
+                    ok[v] = IO(st.template sub_out<Sh::off(k) + int(decltype(v)::value)>());
+                });
             });
             // path levels: written by the first lane of each prefix group
             chen::static_for<P>([&](auto i) {
                 constexpr int k = int(decltype(i)::value);
                 constexpr int grp = int(ipow(W, P - k));
-                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.path[k]);
+                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.template path_out<k>());
             });
         }
     }
@@ -208,12 +216,12 @@ struct StreamCfg {
     static constexpr int kPer = (kRaw + NT - 1) / NT;
 };
 
-template <typename T, int W, int D, int P, typename IO = T, int BLK = 32>
+template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false>
 __global__ __launch_bounds__((StreamCfg<T, W, D, P, BLK>::NT))
 void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
                         const int64_t* __restrict__ begins, const int64_t* __restrict__ ends,
                         const int32_t* __restrict__ order, int64_t n_streams, int64_t tsize,
-                        DevBasis b, uint32_t* __restrict__ counter)
+                        DevBasis b, uint32_t* __restrict__ counter, int flags)
 {
     using Sh = chen::Shape<W, D, P>;
     using C = StreamCfg<T, W, D, P, BLK>;
@@ -282,8 +290,13 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
         const int32_t s = s_stream;
         if (s < 0) break;
         const int64_t row0 = s_row0, len = s_len;
+        if ((flags & kSkipEmpty) && len == 0) {
+            // longest first: every later stream is empty as well
+            if (flags & kSortedDesc) break;
+            continue;
+        }
 
-        chen::Lane<T, W, D, P> st;
+        chen::Lane<T, W, D, P, Comp> st;
         st.init_identity();
         fetch(row0, len, 0);
         store(stage[0]);
@@ -319,21 +332,23 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
                 constexpr int k = P + int(decltype(i)::value);
                 constexpr int nk = int(ipow(W, k - P));
                 IO* ok = o + b.db[k] + int64_t(q) * nk;
-                chen::static_for<nk>([&](auto v) { ok[v] = IO(st.sub[Sh::off(k) + v]); });
+                chen::static_for<nk>([&](auto v) {
+                    ok[v] = IO(st.template sub_out<Sh::off(k) + int(decltype(v)::value)>());
+                });
             });
             chen::static_for<P>([&](auto i) {
                 constexpr int k = int(decltype(i)::value);
                 constexpr int grp = int(ipow(W, P - k));
-                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.path[k]);
+                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.template path_out<k>());
             });
         }
     }
 }
 
-template <typename T, int W, int D, int P, typename IO = T, int WPE = 2>
+template <typename T, int W, int D, int P, typename IO = T, int WPE = 2, bool Comp = false>
 int launch_persistent(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
-                      const int32_t* order,
-                      int64_t n_streams, const DevBasis& b, uint32_t* counter, hipStream_t st)
+                      const int32_t* order, int64_t n_streams, const DevBasis& b,
+                      uint32_t* counter, int flags, hipStream_t st)
 {
     constexpr int NP = chen::Shape<W, D, P>::NP;
     const int64_t n_items = n_streams * NP;
@@ -341,33 +356,33 @@ int launch_persistent(void* out, const void* incr, const int64_t* begins, const 
     RPY_HIP_CHECK(hipGetDevice(&dev));
     RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, chen_persistent_kernel<T, W, D, P, IO, WPE>, kWave, 0));
+            &per_cu, chen_persistent_kernel<T, W, D, P, IO, WPE, Comp>, kWave, 0));
     if (per_cu < 1) per_cu = 1;
     const int64_t chunks = (n_items + kWave - 1) / kWave;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(chunks, int64_t(cus) * per_cu));
-    hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P, IO, WPE>), dim3(uint32_t(grid)), dim3(kWave),
+    hipLaunchKernelGGL((chen_persistent_kernel<T, W, D, P, IO, WPE, Comp>), dim3(uint32_t(grid)), dim3(kWave),
                        0, st, (IO*)out, (const IO*)incr, begins, ends, order, n_items, b.size(), b,
-                       counter);
+                       counter, flags);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }
 
-template <typename T, int W, int D, int P, typename IO = T, int BLK = 32>
+template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false>
 int launch_stream(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
                   const int32_t* order, int64_t n_streams, const DevBasis& b, uint32_t* counter,
-                  hipStream_t st)
+                  int flags, hipStream_t st)
 {
     using C = StreamCfg<T, W, D, P, BLK>;
     int dev = 0, cus = 0, per_cu = 0;
     RPY_HIP_CHECK(hipGetDevice(&dev));
     RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, chen_stream_kernel<T, W, D, P, IO, BLK>, C::NT, 0));
+            &per_cu, chen_stream_kernel<T, W, D, P, IO, BLK, Comp>, C::NT, 0));
     if (per_cu < 1) per_cu = 1;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_streams, int64_t(cus) * per_cu));
-    hipLaunchKernelGGL((chen_stream_kernel<T, W, D, P, IO, BLK>), dim3(uint32_t(grid)), dim3(C::NT),
+    hipLaunchKernelGGL((chen_stream_kernel<T, W, D, P, IO, BLK, Comp>), dim3(uint32_t(grid)), dim3(C::NT),
                        0, st, (IO*)out, (const IO*)incr, begins, ends, order, n_streams, b.size(),
-                       b, counter);
+                       b, counter, flags);
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }

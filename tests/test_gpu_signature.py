@@ -19,6 +19,7 @@ from numpy.testing import assert_allclose, assert_array_equal
 
 import roughpy_amd.compute as rc
 from oracle import oracle as O
+from roughpy_amd import _capi
 from roughpy_amd import streams as rs
 
 pytestmark = pytest.mark.gpu
@@ -195,6 +196,14 @@ def test_benchmark_size_properties(gpu):
     err32 = O.level_errors(a32, a64, w, d).max()
     print(f"benchmark workload f32 vs f64 kernel, all streams: {err32:.3e}")
     assert err32 <= 1e-5
+    # block-compensated float32: the same bars, and below the plain kernel's
+    c32 = rs.signature_batch(dinc, doffs, w, d, compensated=True).cpu().numpy()
+    errc_oracle = O.level_errors(c32[pick], ref, w, d).max()
+    errc = O.level_errors(c32, a64, w, d).max()
+    print(f"benchmark workload compensated f32: vs oracle {errc_oracle:.3e}, "
+          f"vs f64 kernel (all streams) {errc:.3e}")
+    assert errc_oracle <= 1e-5 and errc <= 1e-5
+    assert errc < err32
     # Chen identity: split every stream in two and multiply the halves
     mid = offs[:-1] + lens // 2
     offs2 = np.empty(2 * n + 1, dtype=np.int64)
@@ -208,11 +217,11 @@ def test_benchmark_size_properties(gpu):
     assert O.level_errors(prod.data.cpu().numpy(), a64, w, d).max() <= 1e-12
 
 
-@pytest.mark.parametrize("w,d", [(5, 5), (3, 4), (9, 3)])
-def test_compensated_signatures_round_once(gpu, rng, w, d):
-    """Compensated float32 signatures (float data, float64 arithmetic) equal
-    the float64 oracle on the same float32 increments rounded to float once:
-    normwise level error within 2^-23.  (9, 3) runs on the generic kernel."""
+@pytest.mark.parametrize("w,d", [(9, 3), (3, 7)])
+def test_compensated_generic_shapes_round_once(gpu, rng, w, d):
+    """Shapes without a specialised kernel run the compensated mode in
+    float64 arithmetic on the float32 data: within 2^-23 normwise per level
+    of the float64 oracle on the same float32 increments."""
     lens = rng.integers(1, 300, size=9)
     offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     inc = (rng.normal(size=(int(offs[-1]), w)) / 8).astype(np.float32)
@@ -221,6 +230,129 @@ def test_compensated_signatures_round_once(gpu, rng, w, d):
     for s in range(len(lens)):
         ref = O.signature(inc[offs[s]:offs[s + 1]].astype(np.float64), w, d)
         assert O.level_errors(got[s].astype(np.float64), ref, w, d).max() <= 2.0 ** -23
+
+
+SHAPES = [(3, 4), (5, 5), (8, 4), (6, 5), (4, 6)]
+
+
+def _variant(w, d, dt=None):
+    from roughpy_amd import tune
+    v, _ = tune.config(_capi.RPY_OP_SIGNATURE, w, d,
+                       _capi.RPY_F32_COMPENSATED if dt is None else dt)
+    return v
+
+
+def _long_streams(rng, w, T, n=48):
+    lens = np.concatenate([[40 * T + 3, 16 * T + 7], rng.integers(4 * T, 16 * T + 1, size=n - 2)])
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) /
+           np.sqrt(np.repeat(lens, lens))[:, None]).astype(np.float32)
+    return inc, offs
+
+
+@pytest.mark.parametrize("w,d", SHAPES)
+def test_compensated_lanes_vs_f64_oracle(gpu, rng, w, d):
+    """The compensated mode's float32 lanes (levels 1 and 2 accumulated
+    with Kahan's compensation): the same prefix as the float32 kernel, within
+    1e-5 normwise per level of the float64 oracle on long streams, and on
+    average well below the plain float32 kernel's error.  Per stream the two
+    errors are random quantities of the same kind, so only the worst case
+    and the averages are compared: the compensation removes the error that
+    the lowest levels feed into the higher ones, not the higher levels' own
+    rounding, so the gain is in the average more than in the worst case."""
+    v = _variant(w, d)
+    assert v[0] in (_capi.RPY_KIND_LANE_CHUNK, _capi.RPY_KIND_STREAM_GROUP), v
+    assert v[1] == _variant(w, d, _capi.RPY_F32)[1]
+    inc, offs = _long_streams(rng, w, 256)
+    got = rs.signature_batch(inc, offs, w, d, compensated=True)
+    plain = rs.signature_batch(inc, offs, w, d)
+    ref = O.signature_batch(inc.astype(np.float64), offs, w, d, nthreads=8)
+    e_comp = O.level_errors(got.astype(np.float64), ref, w, d).max(axis=1)
+    e_plain = O.level_errors(plain.astype(np.float64), ref, w, d).max(axis=1)
+    print(f"({w},{d}) compensated lanes: max {e_comp.max():.2e} mean {e_comp.mean():.2e}; "
+          f"plain max {e_plain.max():.2e} mean {e_plain.mean():.2e}")
+    assert e_comp.max() <= 1e-5
+    assert e_comp.mean() <= 0.85 * e_plain.mean()
+
+
+def _block(w, d, T=256):
+    return (_capi.RPY_KIND_BLOCK_MERGE, 0, T, 0)
+
+
+class _Selected:
+    """Runs the block-merge variant of a shape's compensated mode."""
+
+    def __init__(self, w, d, v):
+        self.args = (_capi.RPY_OP_SIGNATURE, w, d, _capi.RPY_F32_COMPENSATED)
+        self.v = v
+
+    def __enter__(self):
+        from roughpy_amd import tune
+        tune.select(*self.args, self.v)
+
+    def __exit__(self, *exc):
+        from roughpy_amd import tune
+        tune.select(*self.args, None)
+
+
+@pytest.mark.parametrize("w,d", SHAPES)
+def test_block_merge_one_segment_is_the_lane_kernel(gpu, rng, w, d):
+    """A stream of at most one segment's rows is the compensated lanes'
+    signature of it, bit for bit (the merge only rounds hi - 0); empty
+    streams give the unit tensor."""
+    T = 256
+    lens = np.concatenate([[0, 1, T], rng.integers(0, T + 1, size=20)])
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / np.sqrt(T)).astype(np.float32)
+    lanes = rs.signature_batch(inc, offs, w, d, compensated=True)
+    with _Selected(w, d, _block(w, d, T)):
+        got = rs.signature_batch(inc, offs, w, d, compensated=True)
+    assert_array_equal(got, lanes)
+    assert got[0, 0] == 1 and not got[0, 1:].any()
+
+
+@pytest.mark.parametrize("w,d", SHAPES)
+def test_block_merge_vs_f64_oracle(gpu, rng, w, d):
+    """Block merge (segments of 256 rows on the compensated lanes, merged
+    per stream with Kahan sums): within 1e-5 of the float64 oracle and on
+    average below the plain float32 kernel's error."""
+    inc, offs = _long_streams(rng, w, 256)
+    with _Selected(w, d, _block(w, d)):
+        got = rs.signature_batch(inc, offs, w, d, compensated=True)
+    plain = rs.signature_batch(inc, offs, w, d)
+    ref = O.signature_batch(inc.astype(np.float64), offs, w, d, nthreads=8)
+    e_comp = O.level_errors(got.astype(np.float64), ref, w, d).max(axis=1)
+    e_plain = O.level_errors(plain.astype(np.float64), ref, w, d).max(axis=1)
+    print(f"({w},{d}) block merge: max {e_comp.max():.2e} mean {e_comp.mean():.2e}; "
+          f"plain max {e_plain.max():.2e} mean {e_plain.mean():.2e}")
+    assert e_comp.max() <= 1e-5, e_comp.max()
+    assert e_comp.mean() <= 0.7 * e_plain.mean()
+
+
+def test_block_merge_rounds_of_streams_agree(gpu, rng):
+    """The allocation-free entry point runs the block merge in rounds of
+    streams that fit its scratch; the result equals the allocating call's."""
+    from roughpy_amd import _device
+    t = gpu
+    w, d = 5, 5
+    lens = rng.integers(0, 3000, size=300)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / 40).astype(np.float32)
+    dinc = t.from_numpy(inc).cuda()
+    doffs = t.from_numpy(offs).cuda()
+    tsize = O.tensor_size(w, d)
+    a = t.empty((300, tsize), dtype=t.float32, device="cuda")
+    b = t.empty_like(a)
+    L = _capi.lib()
+    with _Selected(w, d, _block(w, d)):
+        _capi.check(L.rpy_stream_signature_batch(a.data_ptr(), dinc.data_ptr(), doffs.data_ptr(),
+                                                 300, w, d, _capi.RPY_F32_COMPENSATED,
+                                                 _device.stream_handle()))
+        _capi.check(L.rpy_stream_signature_batch_noalloc(b.data_ptr(), dinc.data_ptr(),
+                                                         doffs.data_ptr(), 300, w, d,
+                                                         _capi.RPY_F32_COMPENSATED,
+                                                         _device.stream_handle()))
+    assert t.equal(a, b)
 
 
 def test_sorted_and_unsorted_schedules_agree(gpu, rng):
