@@ -14,7 +14,7 @@ device memory.
 Layout (all integers and floats little-endian):
 
     magic    8 bytes  b"RPYAMD\\x00\\x01"
-    version  u32      1
+    version  u32      2 (version 1 files, whose caches cover [0, 1), load too)
     kind     u32      1 LieIncrementStream, 2 PiecewiseAbelianStream,
                       3 DyadicSignatureCache
     length   u64      bytes of the body that follows
@@ -25,10 +25,12 @@ Layout (all integers and floats little-endian):
             f64 params[n]; dtype rows[n * width]
     kind 2: i32 width, depth, resolution, 0; f64 support_inf, support_sup;
             i64 n_pieces, lie_dim; f64 infs[n], sups[n], lies[n * lie_dim]
-    kind 3: i32 width, depth, resolution, levels, dtype, 0;
-            i64 n_streams, n_rows, n_nodes_values;
-            i64 offsets[n_streams + 1]; f64 params[n_rows];
+    kind 3: i32 width, depth, resolution, levels, dtype, root_power;
+            i64 n_streams, n_rows, n_nodes_values; f64 origin (version 2);
+            i64 offsets[n_streams + 1]; f64 params[n_rows] (their own units);
             dtype increments[n_rows * width]; dtype nodes[n_nodes_values]
+            (the cache root is [origin, origin + 2^root_power); version 1
+            caches have root_power 0, no origin field and origin 0)
 
 Loading checks the magic, version, kind and body length and raises
 ValueError on a mismatch or a truncated buffer.
@@ -41,7 +43,7 @@ import struct
 import numpy as np
 
 MAGIC = b"RPYAMD\x00\x01"
-VERSION = 1
+VERSION = 2
 KIND_LIE_INCREMENT = 1
 KIND_PIECEWISE_ABELIAN = 2
 KIND_DYADIC_CACHE = 3
@@ -128,12 +130,12 @@ def _cache_parts(c):
     on the device)."""
     from . import _device
     offs = _device.to_host(c._offs).astype(np.int64)
-    params = _device.to_host(c._params).astype(np.float64)
+    params = _device.to_host(c.params()).astype(np.float64)
     inc = _device.to_host(c._inc)
     n_vals = int(c.cache.shape[0])
     head = struct.pack("<iiiiii", c.width, c.depth, c.resolution, c.levels,
-                       _dtype_tag(c.dtype), 0)
-    head += struct.pack("<qqq", c.n_streams, params.shape[0], n_vals)
+                       _dtype_tag(c.dtype), c.root_power)
+    head += struct.pack("<qqqd", c.n_streams, params.shape[0], n_vals, c.origin)
     es = np.dtype(c.dtype).itemsize
     length = len(head) + offs.nbytes + params.nbytes + inc.size * es + n_vals * es
     return _HEADER.pack(MAGIC, VERSION, KIND_DYADIC_CACHE, length) + head, \
@@ -194,11 +196,11 @@ def _read_header(buf: memoryview):
     magic, version, kind, length = _HEADER.unpack_from(buf, 0)
     if magic != MAGIC:
         raise ValueError("not roughpy_amd stream data (bad magic)")
-    if version != VERSION:
+    if version not in (1, VERSION):
         raise ValueError(f"unsupported format version {version}")
     if len(buf) - _HEADER.size != length:
         raise ValueError("stream data length does not match its header")
-    return kind, _Reader(buf[_HEADER.size:])
+    return kind, version, _Reader(buf[_HEADER.size:])
 
 
 def loads(data):
@@ -206,7 +208,7 @@ def loads(data):
     from .streams import DyadicSignatureCache, LieIncrementStream, PiecewiseAbelianStream
     buf = memoryview(bytes(data) if not isinstance(data, (bytes, bytearray, memoryview))
                      else data).cast("B")
-    kind, r = _read_header(buf)
+    kind, version, r = _read_header(buf)
     if kind == KIND_LIE_INCREMENT:
         width, depth, resolution, tag = r.unpack("<iiii")
         sinf, ssup = r.unpack("<dd")
@@ -225,15 +227,19 @@ def loads(data):
         lies = r.array("<f8", n * dim).reshape(n, dim)
         return PiecewiseAbelianStream(infs, sups, lies, width, depth, resolution, (sinf, ssup))
     if kind == KIND_DYADIC_CACHE:
-        width, depth, resolution, levels, tag, _ = r.unpack("<iiiiii")
+        width, depth, resolution, levels, tag, root_power = r.unpack("<iiiiii")
         n_streams, n_rows, n_vals = r.unpack("<qqq")
+        (origin,) = r.unpack("<d") if version >= 2 else (0.0,)
+        if version < 2:
+            root_power = 0
         dt = _tag_dtype(tag)
         offs = r.array("<i8", n_streams + 1)
         params = r.array("<f8", n_rows)
         inc = r.array(dt, n_rows * width).reshape(n_rows, width)
         nodes = r.array(dt, n_vals)
         return DyadicSignatureCache._from_state(inc, params, offs, nodes, width, depth,
-                                                resolution, levels, dt.newbyteorder("="))
+                                                resolution, levels, dt.newbyteorder("="),
+                                                origin, root_power)
     raise ValueError(f"unknown stream data kind {kind}")
 
 

@@ -9,9 +9,10 @@
 * ``signature_batch`` / ``log_signature_batch`` are the batched entry points
   over many uneven streams (CSR offsets), one persistent GPU launch each.
 
-Which increments a query covers is decided on the host by the native
-library (rpy_stream_select_rows); the signature of the covered rows is one
-Chen-product launch on the GPU.
+A signature query at the stream's resolution reads the stream's dyadic
+signature cache (built on the GPU on the first such query and kept); other
+queries, and log-signatures, decide on the host which increments they cover
+(rpy_stream_select_rows, native) and run one Chen-product launch over them.
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ from .compute import DenseFreeTensor, DenseLie
 __all__ = [
     "RealInterval", "Partition", "LieIncrementStream", "PiecewiseAbelianStream",
     "DyadicSignatureCache", "signature_batch", "log_signature_batch",
-    "param_to_resolution", "py_param_to_resolution", "to_dyadic_intervals",
+    "param_to_resolution", "py_param_to_resolution", "to_dyadic_intervals", "cache_root",
 ]
 
 
@@ -370,19 +371,45 @@ def signature_gram(signatures):
     return _device.to_host(G) if isinstance(signatures, np.ndarray) else G
 
 
+def cache_root(inf: float, sup: float, resolution: int):
+    """Root [o, o + 2^k) of a dyadic cache over [inf, sup) at `resolution`:
+    o is inf rounded down to the resolution grid (a multiple of
+    2^-resolution, so every stored node is a union of resolution pieces) and
+    2^k the smallest power of two reaching sup.  Returns (o, k)."""
+    inf, sup = float(inf), float(sup)
+    if not (math.isfinite(inf) and math.isfinite(sup)) or not inf < sup:
+        raise ValueError("a cache support must be a finite interval [inf, sup) with inf < sup")
+    o = math.ldexp(math.floor(math.ldexp(inf, resolution)), -resolution)
+    k = math.ceil(math.log2(sup - o))
+    while o + math.ldexp(1.0, k) < sup:
+        k += 1
+    return o, int(k)
+
+
 class DyadicSignatureCache:
-    """Dyadic signature cache of many streams over [0, 1), built on the GPU.
+    """Dyadic signature cache of many streams, built on the GPU.
 
     increments: (rows, width); params: (rows,) float64, strictly increasing
-    within each stream and in [0, 1); offsets: (n_streams + 1,) CSR offsets.
-    Nodes of powers 0..levels are stored (level-major); ``query`` answers
-    clopen interval queries at ``resolution`` with the semantics of
-    Stream.signature(RealInterval(inf, sup), resolution) for streams of that
-    resolution (streams/src/stream.cpp:107-171).
+    within each stream; offsets: (n_streams + 1,) CSR offsets.  ``support``
+    (inf, sup) is the parameter range the cache covers (default [0, 1)); it
+    is held in a root [o, o + 2^k) (cache_root: o is inf rounded down to the
+    resolution grid), the node of power 0, and nodes of powers 0..levels
+    below it are stored (level-major, power j meaning width 2^(k - j); all
+    are unions of resolution pieces).  Rows outside the root are never
+    covered.  ``query`` answers clopen interval queries at ``resolution``
+    with the semantics of Stream.signature(RealInterval(inf, sup),
+    resolution) for streams of that resolution
+    (streams/src/stream.cpp:107-171): the Chen product over the rows whose
+    param lies in [inf, sup) with both ends rounded down to the resolution
+    grid, at any parameter range as the reference's caching layer answers
+    (streams/src/dyadic_caching_layer.cpp:86-130).  The device kernels work
+    on [0, 1): params and query bounds are mapped by t -> (t - o) 2^-k and
+    the resolution becomes resolution + k; params whose map is not exact
+    are refused (ValueError).
     """
 
     def __init__(self, increments, params, offsets, width: int, depth: int,
-                 resolution: int = 12, levels: int = 6, dtype=None):
+                 resolution: int = 12, levels: int = 6, dtype=None, support=None):
         dt = np.dtype(dtype) if dtype is not None else _device.np_dtype(increments)
         if dt not in (np.float32, np.float64):
             dt = np.dtype(np.float64)
@@ -392,7 +419,8 @@ class DyadicSignatureCache:
         t = _device.torch()
         self._inc = _rows_of(_device.to_device(increments, dt), self.width)
         _check_host_offsets(offsets, int(self._inc.shape[0]))
-        self._params = _device.to_device(params).to(t.float64).contiguous()
+        self._set_root(support)
+        self._params = self._to_unit(_device.to_device(params).to(t.float64).contiguous())
         if int(self._params.shape[0]) != int(self._inc.shape[0]):
             raise ValueError("params must hold one value per increment row")
         self._offs = _device.to_device(np.asarray(offsets, dtype=np.int64)
@@ -407,18 +435,56 @@ class DyadicSignatureCache:
         self.cache = _device.empty((n,), dt)
         self.build()
 
+    def _set_root(self, support):
+        if support is None:
+            self.origin, self.root_power = 0.0, 0
+        else:
+            inf_, sup_ = _bounds_of(support)
+            self.origin, self.root_power = cache_root(inf_, sup_, self.resolution)
+        # resolution of the device kernels, on the root mapped to [0, 1)
+        self._res_unit = self.resolution + self.root_power
+        if not 0 <= self._res_unit <= 30:
+            raise ValueError(f"resolution {self.resolution} on a root of width "
+                             f"2^{self.root_power} is outside what the cache supports "
+                             "(0 <= resolution + log2(root width) <= 30)")
+        if not 0 <= self.levels <= min(self._res_unit, 24):
+            raise ValueError("levels must lie in 0..min(resolution + log2(root width), 24)")
+
+    def _to_unit(self, params):
+        """Params (device float64) on the root mapped to [0, 1); the map is
+        exact, and params outside the root stay outside [0, 1)."""
+        if self.root_power == 0 and self.origin == 0.0:
+            return params
+        scale = math.ldexp(1.0, -self.root_power)
+        unit = (params - self.origin) * scale
+        back = unit * math.ldexp(1.0, self.root_power) + self.origin
+        inside = (unit >= 0) & (unit < 1)
+        if bool(((back != params) & inside).any()):
+            raise ValueError("params are not exactly representable on the cache's root")
+        return unit
+
+    def params(self):
+        """The stream parameters in their own units (device float64)."""
+        if self.root_power == 0 and self.origin == 0.0:
+            return self._params
+        return self._params * math.ldexp(1.0, self.root_power) + self.origin
+
     @classmethod
     def _from_state(cls, increments, params, offsets, nodes, width, depth, resolution, levels,
-                    dtype):
-        """A cache from stored nodes, without rebuilding them."""
+                    dtype, origin=0.0, root_power=0):
+        """A cache from stored nodes, without rebuilding them (params in
+        their own units)."""
         obj = cls.__new__(cls)
         t = _device.torch()
         obj.dtype = np.dtype(dtype)
         obj.width, obj.depth = int(width), int(depth)
         obj.resolution, obj.levels = int(resolution), int(levels)
+        obj.origin, obj.root_power = float(origin), int(root_power)
+        obj._res_unit = obj.resolution + obj.root_power
         obj._inc = _device.to_device(np.ascontiguousarray(increments), obj.dtype).reshape(
             -1, obj.width)
-        obj._params = _device.to_device(np.ascontiguousarray(params, dtype=np.float64))
+        obj._params = obj._to_unit(_device.to_device(np.ascontiguousarray(params,
+                                                                         dtype=np.float64)))
         obj._offs = _device.to_device(np.ascontiguousarray(offsets, dtype=np.int64)).to(t.int64)
         obj.n_streams = int(obj._offs.shape[0]) - 1
         L = _capi.lib()
@@ -453,11 +519,14 @@ class DyadicSignatureCache:
     def build(self):
         _capi.check(_capi.lib().rpy_dyadic_cache_build(
             self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
-            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self.resolution,
+            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self._res_unit,
             self.levels, _dtype_code(self.dtype), _device.stream_handle()))
 
     def node(self, power: int, stream: int, k: int):
-        """Stored signature of stream `stream` over [k 2^-power, (k+1) 2^-power)."""
+        """Stored signature of stream `stream` over the k-th piece of power
+        `power` below the root: [o + k w, o + (k+1) w) with o = self.origin
+        and w = 2^(root_power - power) (for the default [0, 1) root:
+        [k 2^-power, (k+1) 2^-power))."""
         if not (0 <= power <= self.levels and 0 <= k < (1 << power)):
             raise IndexError("node outside the stored levels")
         if not 0 <= stream < self.n_streams:
@@ -485,6 +554,12 @@ class DyadicSignatureCache:
                                if not _device.is_device_array(inf) else inf).to(t.float64).contiguous()
         qs = _device.to_device(np.asarray(sup, dtype=np.float64)
                                if not _device.is_device_array(sup) else sup).to(t.float64).contiguous()
+        if self.root_power != 0 or self.origin != 0.0:
+            # clip to the root, then map to [0, 1)
+            w = math.ldexp(1.0, self.root_power)
+            scale = math.ldexp(1.0, -self.root_power)
+            qi = (qi.clamp(min=self.origin, max=self.origin + w) - self.origin) * scale
+            qs = (qs.clamp(min=self.origin, max=self.origin + w) - self.origin) * scale
         nq = int(sid.shape[0])
         if qi.shape[0] != nq or qs.shape[0] != nq:
             raise ValueError("stream_ids, inf and sup must have the same length")
@@ -502,7 +577,7 @@ class DyadicSignatureCache:
                 raise ValueError("out must be contiguous and on the cache's device")
         _capi.check(_capi.lib().rpy_dyadic_cache_query(
             out.data_ptr(), self.cache.data_ptr(), self._inc.data_ptr(), self._params.data_ptr(),
-            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self.resolution,
+            self._offs.data_ptr(), self.n_streams, self.width, self.depth, self._res_unit,
             self.levels, sid.data_ptr(), qi.data_ptr(), qs.data_ptr(), nq,
             _dtype_code(self.dtype), _device.stream_handle()))
         return out
@@ -522,6 +597,7 @@ class LieIncrementStream:
         self._dev_rows = None
         self._sel_buf = None  # row-selection indices (pinned), reused by simplify
         self._sel_pinned = None
+        self._cache = None    # dyadic signature cache: None not built, False unusable
 
     @classmethod
     def from_increments(cls, data, indices=None, *, width=None, depth=None, resolution=None,
@@ -665,17 +741,51 @@ class LieIncrementStream:
             return (float(inf_), float(sup_)), res
         raise TypeError("unexpected type for argument interval_or_inf")
 
-    def _select(self, interval, resolution):
-        """Row indices the query covers (Stream::refine_interval + dyadic plan)."""
+    def _refine(self, interval, resolution):
+        """(a, b, query resolution) of a query clipped to the support
+        (Stream::refine_interval, streams/src/stream.cpp:52-68), or None when
+        nothing is left."""
         lo, hi = self.support
         inf_, sup_ = interval
         if sup_ < lo or inf_ > hi:
-            return np.zeros(0, dtype=np.int64)
+            return None
         a, b = max(inf_, lo), min(sup_, hi)
         if b - a == 0.0:
-            return np.zeros(0, dtype=np.int64)
+            return None
         q_res = resolution if resolution is not None else max(
             self.resolution, param_to_resolution(b - a) + 2)
+        return a, b, int(q_res)
+
+    def _query_cache(self):
+        """The stream's dyadic signature cache, built on first use and kept:
+        queries at the stream's resolution read it, as the reference's
+        streams memoise their dyadic pieces (DyadicCachingLayer,
+        streams/src/dyadic_caching_layer.cpp:86-130).  The finest stored
+        nodes hold about 32 rows.  None when the stream has no rows or its
+        support and resolution do not fit the cache's range."""
+        if self._cache is None:
+            self._cache = False
+            n = int(self._params.shape[0])
+            lo, hi = self.support
+            if n and hi > lo:
+                try:
+                    _, k = cache_root(lo, hi, self.resolution)
+                    levels = int(math.log2(n / 32)) if n >= 64 else 0
+                    levels = max(0, min(levels, self.resolution + k, 20))
+                    self._cache = DyadicSignatureCache(
+                        self._rows, self._params, np.array([0, n], dtype=np.int64), self.width,
+                        self.depth, resolution=self.resolution, levels=levels, dtype=self.dtype,
+                        support=(lo, hi))
+                except ValueError:
+                    self._cache = False
+        return self._cache or None
+
+    def _select(self, interval, resolution):
+        """Row indices the query covers (Stream::refine_interval + dyadic plan)."""
+        q = self._refine(interval, resolution)
+        if q is None:
+            return np.zeros(0, dtype=np.int64)
+        a, b, q_res = q
         L = _capi.lib()
         n = self._params.shape[0]
         params = np.ascontiguousarray(self._params, dtype=np.float64)
@@ -692,9 +802,16 @@ class LieIncrementStream:
                   dtype=None, ctx=None) -> DenseFreeTensor:
         interval, res = self._parse(interval_or_inf, sup, resolution)
         depth = int(depth if depth is not None else (ctx.depth if ctx is not None else self.depth))
-        rows = self._rows[self._select(interval, res)]
         dt = np.dtype(dtype) if dtype is not None else self.dtype
         basis = TensorBasis(self.width, depth)
+        q = self._refine(interval, res)
+        if q is not None and q[2] == self.resolution and depth == self.depth and dt == self.dtype:
+            cache = self._query_cache()
+            if cache is not None:
+                out = cache.query(np.zeros(1, dtype=np.int64), np.array([q[0]]),
+                                  np.array([q[1]]))
+                return DenseFreeTensor(_device.to_host(out)[0], basis)
+        rows = self._rows[self._select(interval, res)]
         if rows.shape[0] == 0:
             data = np.zeros(basis.size(), dtype=dt)
             data[0] = 1

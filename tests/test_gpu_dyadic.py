@@ -124,3 +124,77 @@ def test_queries_on_the_workgroup_path(gpu, rng):
         sel = (params[offs[s]:offs[s + 1]] >= a[q]) & (params[offs[s]:offs[s + 1]] < b[q])
         ref = O.signature(inc[offs[s]:offs[s + 1]][sel], w, d)
         assert O.level_errors(out[q], ref, w, d).max() <= 1e-12, q
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+@pytest.mark.parametrize("support,r", [((-3.7, 1000.2), 2), ((1e6 + 0.25, 1e6 + 9.5), 4),
+                                       ((0.0, 1000.0), 0)])
+def test_cache_over_any_support(gpu, rng, dtype, tol, support, r):
+    """A cache whose streams live on an arbitrary parameter range (negative,
+    far from zero, default integer indices): nodes below the root are the
+    rows' signatures, and queries equal the oracle's Stream.signature at the
+    stream resolution over the same rows."""
+    w, d, n = 3, 4, 5
+    lo, hi = support
+    grid = np.arange(np.ceil(lo * 2 ** r), np.floor(hi * 2 ** r)) / 2 ** r
+    streams = []
+    for s in range(n):
+        m = int(rng.integers(1, min(400, grid.size) + 1))
+        p = np.sort(rng.choice(grid, size=m, replace=False)) + 2.0 ** -(r + 2)
+        p = p[p < hi]
+        streams.append((p, rng.normal(size=(p.size, w)) / np.sqrt(p.size)))
+    offs = np.concatenate([[0], np.cumsum([p.size for p, _ in streams])]).astype(np.int64)
+    params = np.concatenate([p for p, _ in streams])
+    inc = np.concatenate([x for _, x in streams]).astype(dtype)
+    J = 3
+    cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J, dtype=dtype,
+                                    support=support)
+    o, k = cache.origin, cache.root_power
+    assert o <= lo and o + 2.0 ** k >= hi and o * 2 ** r == np.floor(o * 2 ** r)
+    assert np.array_equal(cache.params().cpu().numpy(), params)
+    inc64 = inc.astype(np.float64)
+    for s in range(n):
+        p, x = params[offs[s]:offs[s + 1]], inc64[offs[s]:offs[s + 1]]
+        for j in range(J + 1):
+            wdt = 2.0 ** (k - j)
+            for kk in range(2 ** j):
+                sel = (p >= o + kk * wdt) & (p < o + (kk + 1) * wdt)
+                got = cache.node(j, s, kk).cpu().numpy()
+                assert O.level_errors(got, O.signature(x[sel], w, d), w, d).max() <= tol
+    nq = 200
+    sid = rng.integers(0, n, size=nq)
+    span = hi - lo
+    a = rng.uniform(lo - 0.1 * span, hi, size=nq)
+    b = a + rng.uniform(0, 0.6 * span, size=nq)
+    a[:3], b[:3] = [lo, lo, hi], [hi, lo, hi + 1]
+    out = cache.query(sid, a, b).cpu().numpy()
+    for q in range(nq):
+        s = sid[q]
+        st = O.IncrementStream(inc64[offs[s]:offs[s + 1]], w, d,
+                               indices=params[offs[s]:offs[s + 1]], resolution=r)
+        lo_q, hi_q = max(a[q], o), min(b[q], o + 2.0 ** k)
+        if hi_q <= lo_q:
+            ref = np.eye(1, O.tensor_size(w, d)).ravel()
+        else:
+            ref = st.signature(lo_q, hi_q, r)
+        assert O.level_errors(out[q], ref, w, d).max() <= tol, (q, s, a[q], b[q])
+
+
+def test_stream_queries_read_the_cache(gpu, rng):
+    """LieIncrementStream.signature at the stream resolution goes through the
+    stream's dyadic cache (built once, kept); other resolutions select rows
+    on the host; both agree with the oracle."""
+    w, d, n = 3, 4, 1000
+    inc = rng.normal(0.0, np.sqrt(1.0 / n), size=(n, w))
+    s = rs.LieIncrementStream.from_increments(inc, width=w, depth=d)
+    so = O.IncrementStream(inc, w, d)
+    for a, b in [(0, 1000), (10.5, 500.25), (999, 1000), (123, 124), (-5, 3), (-5, 2000)]:
+        got = s.signature(a, b).data
+        assert O.level_errors(got, so.signature(a, b), w, d).max() <= 1e-12, (a, b)
+    cache = s._query_cache()
+    assert cache is not None and cache.origin == 0.0 and cache.root_power == 10
+    assert cache.levels == 4
+    assert s._query_cache() is cache
+    # a finer query resolution takes the host path, with the same answer
+    got = s.signature(rs.RealInterval(10.0, 500.0), 3).data
+    assert O.level_errors(got, so.signature(10.0, 500.0, 3), w, d).max() <= 1e-12

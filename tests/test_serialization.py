@@ -36,7 +36,7 @@ def test_byte_layout_known_answer():
     s = rs.LieIncrementStream.from_increments(data, width=3, depth=4)
     body = struct.pack("<iiii", 3, 4, 0, 8) + struct.pack("<dd", 0.0, 2.0) + struct.pack("<q", 2)
     body += struct.pack("<2d", 0.0, 1.0) + struct.pack("<6d", *data.ravel())
-    expected = b"RPYAMD\x00\x01" + struct.pack("<IIQ", 1, 1, len(body)) + body
+    expected = b"RPYAMD\x00\x01" + struct.pack("<IIQ", 2, 1, len(body)) + body
     assert ser.dumps(s) == expected
 
 
