@@ -4,9 +4,12 @@ Modules:
   compute  -- drop-in for roughpy.compute (ft_fma, ft_exp, ft_log, ...)
   streams  -- LieIncrementStream and batched signatures / log-signatures
   bases    -- TensorBasis, LieBasis (Hall basis), sparse l2t / t2l maps
+  algebra  -- the main library's FreeTensor, ShuffleTensor, Lie, Context and
+              get_context objects over the same kernels
 
-All compute runs in hand-written HIP kernels in _lib/libroughpy_amd.so; the
-package never computes on the CPU.
+All algebra (products, exp, log, maps, signatures) runs in hand-written HIP
+kernels in _lib/libroughpy_amd.so; only sums and scalings of single host
+elements are done on the host.
 """
 from . import _capi
 from .bases import LieBasis, SparseMatrix, TensorBasis, check_lie_basis_data
