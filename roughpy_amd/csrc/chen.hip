@@ -317,9 +317,16 @@ int plan_signature(const void* out, const void* increments, const int64_t* begin
     p.generic_grid = int(std::min<int64_t>(n_streams, max_generic_grid));
     p.seg_var = nullptr;
     if (p.var && p.var->v.kind == RPY_KIND_BLOCK_MERGE) {
-        // segments run on the compensated lanes (the default variant)
-        p.seg_var = find_in(registry(), width, depth, RPY_F32_COMPENSATED,
-                            default_variant(width, depth, RPY_F32_COMPENSATED));
+        // segments run on the compensated lanes a compensated call runs
+        // without a selection: the tuned row when it is a lane variant, else
+        // the default
+        rpy_kernel_variant lanes = default_variant(width, depth, RPY_F32_COMPENSATED);
+        rpy_kernel_variant t;
+        if (tuned_variant(RPY_OP_SIGNATURE, width, depth, RPY_F32_COMPENSATED, &t) &&
+            (t.kind == RPY_KIND_LANE_CHUNK || t.kind == RPY_KIND_STREAM_GROUP) &&
+            find_in(registry(), width, depth, RPY_F32_COMPENSATED, t))
+            lanes = t;
+        p.seg_var = find_in(registry(), width, depth, RPY_F32_COMPENSATED, lanes);
         if (!p.seg_var || !p.seg_var->launch)
             return fail(RPY_ERR_UNSUPPORTED, "no float32 signature kernel for width %d depth %d",
                         width, depth);
