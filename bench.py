@@ -230,6 +230,23 @@ def fma_leg(torch, steps, warmup):
     return res
 
 
+def hbm_leg(torch, steps, warmup):
+    """What plain device copies reach on this GPU's HBM, beside the 8 TB/s
+    nominal figure the HBM rooflines are quoted against: torch's copy_
+    (read and write, 1:1) and fill_ (write only) over 4 GiB."""
+    nbytes = 4 << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    copy_ms = time_kernel(lambda: b.copy_(a), steps, warmup, torch)
+    fill_ms = time_kernel(lambda: b.fill_(2.0), steps, warmup, torch)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"workload": "4 GiB float32, torch copy_ and fill_",
+            "copy_tb_s": 2 * nbytes / (copy_ms * 1e-3) / 1e12,
+            "fill_tb_s": nbytes / (fill_ms * 1e-3) / 1e12,
+            "nominal_tb_s": PEAK_HBM_GBS / 1e3}
+
+
 def dense_leg(torch, steps, warmup):
     """The other dense operations of the path on a batch of 16384 tensors of
     width 5, depth 5 (3906 coefficients; Lie dimension 829), inputs resident
@@ -798,10 +815,10 @@ def main():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--legs",
-                    default="dyadic,fma,dense,backward,logsig,gram,shuffle,simplify,serialize,"
-                            "graph",
+                    default="dyadic,hbm,fma,dense,backward,logsig,gram,shuffle,simplify,"
+                            "serialize,graph",
                     help="comma-separated secondary workloads to run after the headline "
-                         "(dyadic, fma, dense, backward, logsig, gram, shuffle, simplify, "
+                         "(dyadic, hbm, fma, dense, backward, logsig, gram, shuffle, simplify, "
                          "serialize, graph; empty for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy, the reference "
@@ -914,6 +931,9 @@ def main():
         progress("dyadic_cache done")
     del dinc64, results, s32, s64, out
     torch.cuda.empty_cache()
+    if "hbm" in legs:
+        line["hbm_copy"] = hbm_leg(torch, max(3, args.steps // 2), 1)
+        progress("hbm_copy done")
     if "fma" in legs:
         line["fma_w4_d6"] = fma_leg(torch, max(3, args.steps // 2), 1)
         progress("fma_w4_d6 done")

@@ -301,10 +301,10 @@ def signature_batch(increments, offsets, width: int, depth: int, dtype=None,
     increments: (rows, width) array (NumPy or GPU tensor); offsets: (n+1,)
     int64 CSR offsets, stream s owning rows offsets[s]:offsets[s+1].
     Returns an (n, tensor_size) GPU tensor (NumPy if increments is NumPy).
-    compensated (float32 data only): at the benchmark shapes, float32
-    signatures of segments of the streams merged per stream with
-    compensated float32 sums (smaller error than plain float32 on long
-    streams); elsewhere float64 arithmetic on the float32 data.
+    compensated (float32 data only): the float32 kernels with signature
+    levels 1 and 2 accumulated by Kahan's compensated summation (smaller
+    error than plain float32 on long streams); shapes without a fixed-shape
+    kernel use float64 arithmetic on the float32 data, rounded once.
     """
     dt = np.dtype(dtype) if dtype is not None else _device.np_dtype(increments)
     if dt not in (np.float32, np.float64):

@@ -106,6 +106,64 @@ def test_cache_benchmark_shape_sample(gpu):
         assert O.level_errors(out[q], ref, w, d).max() <= 2e-5
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [40, 3000])
+def test_cache_leaves_at_the_benchmark_shape(gpu, rng, dtype, n):
+    """At width 5, depth 5 (stored levels 6) each level-J leaf is the
+    signature of its segment's rows as the batched signature call computes
+    it (float32 bit for bit; float64 within 1e-13), including empty
+    streams, empty leading, inner and trailing segments, and rows outside
+    [0, 1); 3000 streams take the length-sorted queue.  A sample of nodes of
+    every level is checked against the oracle."""
+    w, d, r, J = 5, 5, 12, 6
+    lens = rng.integers(0, 200, size=n)
+    lens[:4] = [0, 1, 2, 150]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    params = []
+    for i, L in enumerate(lens):
+        p = np.sort(rng.uniform(-0.05, 1.05, size=L)) if i % 3 == 0 else np.arange(L) / max(L, 1)
+        if i % 5 == 1:
+            p = 0.3 + 0.1 * p    # every row in a few segments: empty ones around them
+        params.append(p)
+    params = np.concatenate(params).astype(np.float64)
+    inc = (rng.normal(size=(int(offs[-1]), w)) / 8).astype(dtype)
+    cache = rs.DyadicSignatureCache(inc, params, offs, w, d, resolution=r, levels=J, dtype=dtype)
+    # segment row ranges: rows with params in [k 2^-J, (k+1) 2^-J)
+    nseg = 2 ** J
+    bounds = np.empty((n, nseg + 1), dtype=np.int64)
+    for s in range(n):
+        p = params[offs[s]:offs[s + 1]]
+        bounds[s] = offs[s] + np.searchsorted(p, np.arange(nseg + 1) / nseg, side="left")
+    seg_offs_b = bounds[:, :-1].ravel()
+    seg_offs_e = bounds[:, 1:].ravel()
+    # the same segments through the batched signature call (rows gathered per segment)
+    rows = np.concatenate([np.arange(b0, e0) for b0, e0 in zip(seg_offs_b, seg_offs_e)])
+    seg_offs = np.concatenate([[0], np.cumsum(seg_offs_e - seg_offs_b)]).astype(np.int64)
+    ref = rs.signature_batch(inc[rows], seg_offs, w, d)
+    ts = O.tensor_size(w, d)
+    first = (nseg - 1) * n * ts
+    leaves_all = cache.cache[first:first + n * nseg * ts].cpu().numpy().reshape(n * nseg, ts)
+    leaves = np.stack([cache.node(J, s, k).cpu().numpy() for s in range(3) for k in range(nseg)])
+    if dtype == np.float32:
+        np.testing.assert_array_equal(leaves_all, ref)
+    else:
+        err = O.level_errors(leaves_all, ref, w, d).max()
+        assert err <= 1e-13, err
+    assert np.array_equal(leaves, leaves_all[:len(leaves)])
+    # a sample of every level against the oracle
+    inc64 = inc.astype(np.float64)
+    tol = 2e-5 if dtype == np.float32 else 1e-12
+    for s in range(6):
+        p = params[offs[s]:offs[s + 1]]
+        x = inc64[offs[s]:offs[s + 1]]
+        for j in range(J + 1):
+            for k in sorted({0, 2 ** j - 1, (2 ** j) // 3}):
+                sel = (p >= k / 2 ** j) & (p < (k + 1) / 2 ** j)
+                want = O.signature(x[sel], w, d)
+                got = cache.node(j, s, k).cpu().numpy()
+                assert O.level_errors(got, want, w, d).max() <= tol, (s, j, k)
+
+
 def test_queries_on_the_workgroup_path(gpu, rng):
     """Width 2, depth 9 has no query kernel of its own, so its queries take
     the all-workgroup kernel; same oracle check."""
