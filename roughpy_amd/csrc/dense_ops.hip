@@ -1797,8 +1797,12 @@ int dense_sparse_map(void* out, const void* arg, const rpy_sparse_matrix* m, rpy
         auto go = [&](auto tag, auto ec) {
             using T = decltype(tag);
             constexpr int EE = decltype(ec)::value;
-            constexpr int BL = EE == 8 ? 256 : 512;
-            auto* kern = csr_map_lds_kernel<T, EE, BL>;
+            // 512 threads; 1024 when the staged tensors take more than 64 KiB
+            // (one workgroup per CU): t2l (5,5) float64 365 -> 257 us, l2t
+            // float32 171 -> 150 us (E = 8 at 256 -> 512 threads), 16384 tensors
+            const bool big = lds > 64 * 1024;
+            auto* kern = big ? csr_map_lds_kernel<T, EE, 1024> : csr_map_lds_kernel<T, EE, 512>;
+            const int BL = big ? 1024 : 512;
             if (lds > 64 * 1024) {
                 static std::atomic<bool> reserved{false};
                 if (!reserved.load(std::memory_order_acquire)) {
