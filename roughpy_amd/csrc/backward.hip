@@ -321,6 +321,122 @@ __global__ __launch_bounds__((chen::Shape<W, D, P>::NP)) void sig_vjp_wave_kerne
     }
 }
 
+// Stream-group backward for splits whose lane groups do not align with
+// power-of-two lane boundaries (widths 3, 5, 6, 7): a workgroup of W^P lanes
+// (ceil(W^P / 64) waves) per stream, the same lane algebra as the wave
+// kernel, and the cross-lane sums through LDS with two barriers per step,
+// each in a fixed order (deterministic):
+//   * path cotangents: the lanes write their shares; one thread per group of
+//     W lanes sharing u_0..u_{P-2} sums the group's W shares of every path
+//     level; each lane then adds the W^(P-1-k) group sums below its level-k
+//     path word;
+//   * gradient: the W values are summed over each wave by a reduce-scatter
+//     (W padded to 8), then over the waves by W threads.
+template <int W, int P>
+struct VjpGroupCfg {
+    static constexpr int NP = int(ipow(W, P));
+    static constexpr int NW = (NP + 63) / 64;
+    static constexpr int NT = NW * 64;
+    static constexpr int NG = int(ipow(W, P - 1));  // groups of W lanes
+};
+
+template <typename T, int W, int D, int P>
+__global__ __launch_bounds__((VjpGroupCfg<W, P>::NT)) void sig_vjp_group_kernel(
+        T* __restrict__ grad, const T* __restrict__ incr, const int64_t* __restrict__ offsets,
+        const T* __restrict__ sig, const T* __restrict__ gsig, DevBasis b)
+{
+    using C = VjpGroupCfg<W, P>;
+    constexpr int NP = C::NP, NW = C::NW, NT = C::NT, NG = C::NG;
+    static_assert(W <= 8 && P >= 2 && NG <= NT, "group backward shape");
+    __shared__ T s_dp[2][P][NT];  // path cotangent shares (levels 1..P-1)
+    __shared__ T s_hs[2][P][NG];  // their sums over each group of W lanes
+    __shared__ T s_g[2][NW][8];   // wave totals of the gradient
+    const int64_t s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const bool active = tid < NP;
+    const int q = active ? tid : 0;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const int64_t tsize = b.size();
+    int u[P];
+    prefix_digits<W, P>(q, u);
+    chen::BwdLane<T, W, D, P> st;
+    st.set_prefix(u);
+    load_share<T, W, D, P>(st.A.path, st.A.sub, sig + s * tsize, b, q);
+    load_share<T, W, D, P>(st.hpath, st.hsub, gsig + s * tsize, b, q);
+    const int64_t r0 = offsets[s], r1 = offsets[s + 1];
+    int par = 0;
+    for (int64_t row = r1 - 1; row >= r0; --row) {
+        const T* xr = incr + row * W;
+        T tab[D * W];
+        chen::build_table<T, W, D>(tab, xr);
+        T pu[P];
+        for (int j = 0; j < P; ++j) pu[j] = xr[u[j]];
+        st.A.step(chen::RegRow<T, W, D, P, true>{tab, pu});
+        st.contributions(chen::RegRow<T, W, D, P, false>{tab, pu});
+        // shares of the path cotangents
+        chen::static_for<P - 1>([&](auto i) {
+            constexpr int k = int(decltype(i)::value) + 1;
+            s_dp[par][k][tid] = active ? st.dpath[k] : T(0);
+        });
+        // gradient: wave reduce-scatter of the W values (padded to 8); lane
+        // ends with the wave total of letter lane >> 3 when lane % 8 == 0
+        T v[8];
+        chen::static_for<8>([&](auto a) {
+            constexpr int ai = int(decltype(a)::value);
+            if constexpr (ai < W) {
+                T x = st.g[ai];
+                chen::static_for<P>([&](auto j) {
+                    if (u[decltype(j)::value] == ai) x += st.gp[decltype(j)::value];
+                });
+                v[ai] = active ? x : T(0);
+            } else {
+                v[ai] = T(0);
+            }
+        });
+        chen::static_for<3>([&](auto i0) {
+            constexpr int i = int(decltype(i0)::value);
+            constexpr int half = 4 >> i;
+            constexpr int sft = 32 >> i;
+            const bool upper = (lane & sft) != 0;
+            chen::static_for<half>([&](auto k0) {
+                constexpr int k = int(decltype(k0)::value);
+                const T send = upper ? v[k] : v[k + half];
+                const T keep = upper ? v[k + half] : v[k];
+                v[k] = keep + __shfl_xor(send, sft);
+            });
+        });
+        T tot = v[0];
+        for (int sft = 4; sft >= 1; sft >>= 1) tot += __shfl_xor(tot, sft);
+        if ((lane & 7) == 0) s_g[par][wave][lane >> 3] = tot;
+        __syncthreads();
+        if (tid < NG) {
+            chen::static_for<P - 1>([&](auto i) {
+                constexpr int k = int(decltype(i)::value) + 1;
+                T acc = T(0);
+                for (int c = 0; c < W; ++c) acc += s_dp[par][k][tid * W + c];
+                s_hs[par][k][tid] = acc;
+            });
+        }
+        if (tid < W) {
+            T g = T(0);
+            for (int w = 0; w < NW; ++w) g += s_g[par][w][tid];
+            grad[row * W + tid] = g;
+        }
+        __syncthreads();
+        // level-k path word of this lane: groups g0 .. g0 + W^(P-1-k) - 1
+        chen::static_for<P - 1>([&](auto i) {
+            constexpr int k = int(decltype(i)::value) + 1;
+            constexpr int span = int(ipow(W, P - 1 - k));
+            const int g0 = (q / W / span) * span;
+            T acc = T(0);
+            for (int c = 0; c < span; ++c) acc += s_hs[par][k][g0 + c];
+            st.hpath[k] += acc;
+        });
+        par ^= 1;
+    }
+}
+
 // Host run of the wave kernel's lane algebra for one float64 stream (the
 // cross-lane sums in plain loops).
 template <int W, int D, int P>
@@ -381,6 +497,15 @@ void launch_wave(void* grad, const void* incr, const int64_t* offsets, int64_t n
                        (T*)grad, (const T*)incr, offsets, (const T*)sig, (const T*)gsig, b);
 }
 
+template <typename T, int W, int D, int P>
+void launch_group(void* grad, const void* incr, const int64_t* offsets, int64_t n,
+                  const void* sig, const void* gsig, const DevBasis& b, hipStream_t st)
+{
+    hipLaunchKernelGGL((sig_vjp_group_kernel<T, W, D, P>), dim3(uint32_t(n)),
+                       dim3(VjpGroupCfg<W, P>::NT), 0, st, (T*)grad, (const T*)incr, offsets,
+                       (const T*)sig, (const T*)gsig, b);
+}
+
 // Register-resident backward variants (kind RPY_KIND_WAVE_VJP): a
 // workgroup of W^P lanes per stream.  The defaults are 64-lane splits (one
 // wave per stream): for float64 at (8,4), (4,6) and (2,10) the lane state
@@ -395,6 +520,7 @@ using VjpLauncher = void (*)(void* grad, const void* incr, const int64_t* offset
 struct VjpVariant {
     int32_t width, depth;
     rpy_dtype dtype;
+    int32_t kind;  // RPY_KIND_WAVE_VJP or RPY_KIND_STREAM_GROUP
     int32_t prefix;
     bool is_default;
     VjpLauncher launch;
@@ -403,8 +529,18 @@ struct VjpVariant {
 template <int W, int D, int P, bool Default>
 void add_wave(std::vector<VjpVariant>& v)
 {
-    v.push_back({W, D, RPY_F32, P, Default, &launch_wave<float, W, D, P>});
-    v.push_back({W, D, RPY_F64, P, Default, &launch_wave<double, W, D, P>});
+    v.push_back({W, D, RPY_F32, RPY_KIND_WAVE_VJP, P, Default, &launch_wave<float, W, D, P>});
+    v.push_back({W, D, RPY_F64, RPY_KIND_WAVE_VJP, P, Default, &launch_wave<double, W, D, P>});
+}
+
+// Stream-group backward (sig_vjp_group_kernel), kind RPY_KIND_STREAM_GROUP.
+template <int W, int D, int P, bool Default>
+void add_group(std::vector<VjpVariant>& v)
+{
+    v.push_back({W, D, RPY_F32, RPY_KIND_STREAM_GROUP, P, Default,
+                 &launch_group<float, W, D, P>});
+    v.push_back({W, D, RPY_F64, RPY_KIND_STREAM_GROUP, P, Default,
+                 &launch_group<double, W, D, P>});
 }
 
 const std::vector<VjpVariant>& vjp_registry()
@@ -425,21 +561,25 @@ const std::vector<VjpVariant>& vjp_registry()
         add_wave<2, 10, 6, true>(v);
         add_wave<8, 4, 3, false>(v);
         add_wave<4, 6, 4, false>(v);
+        add_group<5, 5, 3, true>(v);
+        add_group<6, 5, 3, true>(v);
         return v;
     }();
     return r;
 }
 
-rpy_kernel_variant wave_variant(int32_t prefix)
+rpy_kernel_variant vjp_kind(const VjpVariant& x)
 {
-    return rpy_kernel_variant{RPY_KIND_WAVE_VJP, prefix, 0, 0};
+    return rpy_kernel_variant{x.kind, x.prefix, 0, 0};
 }
 
 const VjpVariant* find_vjp(int32_t w, int32_t d, rpy_dtype dt, const rpy_kernel_variant& k)
 {
-    if (k.kind != RPY_KIND_WAVE_VJP || k.block != 0 || k.waves_per_eu != 0) return nullptr;
+    if (k.block != 0 || k.waves_per_eu != 0) return nullptr;
     for (const VjpVariant& x : vjp_registry())
-        if (x.width == w && x.depth == d && x.dtype == dt && x.prefix == k.prefix) return &x;
+        if (x.width == w && x.depth == d && x.dtype == dt && x.kind == k.kind &&
+            x.prefix == k.prefix)
+            return &x;
     return nullptr;
 }
 
@@ -454,8 +594,7 @@ const VjpVariant* choose_vjp(int32_t w, int32_t d, rpy_dtype dt, int32_t* source
 {
     rpy_kernel_variant def{RPY_KIND_GENERIC, 0, 0, 0};
     for (const VjpVariant& x : vjp_registry())
-        if (x.width == w && x.depth == d && x.dtype == dt && x.is_default)
-            def = wave_variant(x.prefix);
+        if (x.width == w && x.depth == d && x.dtype == dt && x.is_default) def = vjp_kind(x);
     const rpy_kernel_variant k = resolve_variant(
             RPY_OP_SIGNATURE_VJP, w, d, dt, def,
             [&](const rpy_kernel_variant& v) { return vjp_exists(w, d, dt, v); }, source);
@@ -602,7 +741,7 @@ int32_t vjp_variants(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_v
     int32_t n = 0;
     for (const VjpVariant& x : vjp_registry()) {
         if (x.width != width || x.depth != depth || x.dtype != dtype) continue;
-        if (out && n < max) out[n] = wave_variant(x.prefix);
+        if (out && n < max) out[n] = vjp_kind(x);
         ++n;
     }
     if (out && n < max) out[n] = rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0};
@@ -627,7 +766,7 @@ int vjp_config(int32_t width, int32_t depth, rpy_dtype dtype, rpy_kernel_variant
     if (!out) return fail(RPY_ERR_INVALID_ARGUMENT, "null pointer");
     if (dtype != RPY_F32 && dtype != RPY_F64) return fail(RPY_ERR_DTYPE, "unsupported dtype");
     const VjpVariant* v = choose_vjp(width, depth, dtype, source);
-    *out = v ? wave_variant(v->prefix) : rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0};
+    *out = v ? vjp_kind(*v) : rpy_kernel_variant{RPY_KIND_GENERIC, 0, 0, 0};
     return RPY_OK;
 }
 

@@ -116,7 +116,7 @@ def test_call_runs_the_configured_variant(gpu, w, d):
         assert torch.equal(plain, forced), (dt, v)
 
 
-@pytest.mark.parametrize("w,d", [(8, 4), (4, 6)])
+@pytest.mark.parametrize("w,d", [(8, 4), (4, 6), (5, 5), (6, 5)])
 def test_every_backward_variant_matches_the_generic_backward(gpu, w, d):
     torch = gpu
     rng = np.random.default_rng(31 + w + d)
@@ -147,7 +147,7 @@ def test_every_backward_variant_matches_the_generic_backward(gpu, w, d):
         for dt, tdt, tol in ((_capi.RPY_F64, torch.float64, 1e-10),
                              (_capi.RPY_F32, torch.float32, 1e-4)):
             vs = _variants(op, w, d, dt)
-            assert len(vs) >= 3
+            assert len(vs) >= 2
             for v in vs:
                 print(f"backward {w}x{d} dtype {dt} variant {v}", flush=True)
                 _select(op, w, d, dt, v)
