@@ -384,6 +384,42 @@ def log_signature_leg(torch, steps, warmup):
     return res
 
 
+def headline_backward_leg(torch, dinc32, doffs, steps, warmup):
+    """Reverse-Chen backward of the headline batch (16384 streams of width 5,
+    lengths U{64..4096}, depth 5): loss <G, S> with a seeded random G, the
+    same four Chen-step equivalents per increment as the width-8 leg."""
+    from roughpy_amd import _capi, _device
+    w, d = W, D
+    n = int(doffs.shape[0]) - 1
+    rows = int(dinc32.shape[0])
+    lib = _capi.lib()
+    tsize = int(lib.rpy_tensor_size(w, d))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 41)
+    G32 = torch.randn((n, tsize), generator=gen, device="cuda")
+    fma_step = sum((d - m + 1) * w ** m for m in range(1, d + 1))
+    res = {"workload": f"backward of the headline batch: {n} streams, width {w}, depth {d}, "
+                       f"{rows} increments"}
+    for name, code, inc, G in (("f32", _capi.RPY_F32, dinc32, G32),
+                               ("f64", _capi.RPY_F64, dinc32.double(), G32.double())):
+        S = torch.empty((n, tsize), dtype=inc.dtype, device="cuda")
+        grad = torch.empty_like(inc)
+        _capi.check(lib.rpy_stream_signature_batch(S.data_ptr(), inc.data_ptr(), doffs.data_ptr(),
+                                                   n, w, d, code, _device.stream_handle()))
+
+        def bwd():
+            _capi.check(lib.rpy_stream_signature_vjp(grad.data_ptr(), inc.data_ptr(),
+                                                     doffs.data_ptr(), n, w, d, S.data_ptr(),
+                                                     G.data_ptr(), code, _device.stream_handle()))
+        ms = time_kernel(bwd, steps, warmup, torch)
+        tf = 2.0 * 4 * fma_step * rows / (ms * 1e-3) / 1e12
+        res[name] = {"backward_ms": ms, "gradients_per_s": n / (ms * 1e-3),
+                     "backward_tflops": tf, "backward_frac": tf / PEAK_TFLOPS[name]}
+        del S, grad
+    torch.cuda.empty_cache()
+    return res
+
+
 def backward_leg(torch, steps, warmup):
     """Signature forward + reverse-Chen backward, width 8, depth 4 (4681
     coefficients), 4096 streams of length 256, N(0, 1/256) increments, loss
@@ -929,7 +965,14 @@ def main():
     if "dyadic" in legs:
         line["dyadic_cache"] = dyadic_cache_leg(torch, inc, offs, lens, dinc32, doffs)
         progress("dyadic_cache done")
-    del dinc64, results, s32, s64, out
+    if "backward" in legs:
+        del dinc64, s64
+        torch.cuda.empty_cache()
+        line["backward_w5_d5"] = headline_backward_leg(torch, dinc32, doffs, max(2, args.steps // 4), 1)
+        progress("backward_w5_d5 done")
+    del results, s32, out
+    if "backward" not in legs:
+        del dinc64, s64
     torch.cuda.empty_cache()
     if "hbm" in legs:
         line["hbm_copy"] = hbm_leg(torch, max(3, args.steps // 2), 1)

@@ -561,6 +561,7 @@ const std::vector<VjpVariant>& vjp_registry()
         add_wave<2, 10, 6, true>(v);
         add_wave<8, 4, 3, false>(v);
         add_wave<4, 6, 4, false>(v);
+        add_group<3, 4, 3, true>(v);
         add_group<5, 5, 3, true>(v);
         add_group<6, 5, 3, true>(v);
         return v;

@@ -116,7 +116,7 @@ def test_call_runs_the_configured_variant(gpu, w, d):
         assert torch.equal(plain, forced), (dt, v)
 
 
-@pytest.mark.parametrize("w,d", [(8, 4), (4, 6), (5, 5), (6, 5)])
+@pytest.mark.parametrize("w,d", [(8, 4), (4, 6), (5, 5), (6, 5), (3, 4)])
 def test_every_backward_variant_matches_the_generic_backward(gpu, w, d):
     torch = gpu
     rng = np.random.default_rng(31 + w + d)
