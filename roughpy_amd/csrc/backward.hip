@@ -332,6 +332,9 @@ __global__ __launch_bounds__((chen::Shape<W, D, P>::NP)) void sig_vjp_wave_kerne
 //     path word;
 //   * gradient: the W values are summed over each wave by a reduce-scatter
 //     (W padded to 8), then over the waves by W threads.
+// The path cotangents are only read by the path targets, so each step runs
+// the prefix recovery and the subtree targets before it waits for the
+// previous step's group sums: the group summing overlaps that work.
 template <int W, int P>
 struct VjpGroupCfg {
     static constexpr int NP = int(ipow(W, P));
@@ -373,7 +376,20 @@ __global__ __launch_bounds__((VjpGroupCfg<W, P>::NT)) void sig_vjp_group_kernel(
         T pu[P];
         for (int j = 0; j < P; ++j) pu[j] = xr[u[j]];
         st.A.step(chen::RegRow<T, W, D, P, true>{tab, pu});
-        st.contributions(chen::RegRow<T, W, D, P, false>{tab, pu});
+        st.contributions_sub(chen::RegRow<T, W, D, P, false>{tab, pu});
+        if (row != r1 - 1) {
+            __syncthreads();  // the previous step's group sums are written
+            // level-k path word of this lane: groups g0 .. g0 + W^(P-1-k) - 1
+            chen::static_for<P - 1>([&](auto i) {
+                constexpr int k = int(decltype(i)::value) + 1;
+                constexpr int span = int(ipow(W, P - 1 - k));
+                const int g0 = (q / W / span) * span;
+                T acc = T(0);
+                for (int c = 0; c < span; ++c) acc += s_hs[par ^ 1][k][g0 + c];
+                st.hpath[k] += acc;
+            });
+        }
+        st.contributions_path(chen::RegRow<T, W, D, P, false>{tab, pu});
         // shares of the path cotangents
         chen::static_for<P - 1>([&](auto i) {
             constexpr int k = int(decltype(i)::value) + 1;
@@ -423,16 +439,6 @@ __global__ __launch_bounds__((VjpGroupCfg<W, P>::NT)) void sig_vjp_group_kernel(
             for (int w = 0; w < NW; ++w) g += s_g[par][w][tid];
             grad[row * W + tid] = g;
         }
-        __syncthreads();
-        // level-k path word of this lane: groups g0 .. g0 + W^(P-1-k) - 1
-        chen::static_for<P - 1>([&](auto i) {
-            constexpr int k = int(decltype(i)::value) + 1;
-            constexpr int span = int(ipow(W, P - 1 - k));
-            const int g0 = (q / W / span) * span;
-            T acc = T(0);
-            for (int c = 0; c < span; ++c) acc += s_hs[par][k][g0 + c];
-            st.hpath[k] += acc;
-        });
         par ^= 1;
     }
 }

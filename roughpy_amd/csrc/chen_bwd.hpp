@@ -95,6 +95,26 @@ struct BwdLane {
         static_for<D - P + 1>([&](auto i) { target<P + int(decltype(i)::value)>(acc); });
     }
 
+    // contributions() in two parts for callers that update hpath in
+    // between: the subtree targets (which do not read hpath; they also clear
+    // the step's accumulators), then the path targets.  The shares are
+    // accumulated in a different order than by contributions().
+    template <typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void contributions_sub(const Acc& acc)
+    {
+        static_for<W>([&](auto a) { g[decltype(a)::value] = T(0); });
+        static_for<P>([&](auto j) {
+            gp[decltype(j)::value] = T(0);
+            dpath[decltype(j)::value] = T(0);
+        });
+        static_for<D - P + 1>([&](auto i) { target<P + int(decltype(i)::value)>(acc); });
+    }
+    template <typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void contributions_path(const Acc& acc)
+    {
+        static_for<P - 1>([&](auto i) { path_target<int(decltype(i)::value) + 1>(acc); });
+    }
+
     // this lane's share of dl/dx_a (before the cross-lane sum)
     RPY_HD inline __attribute__((always_inline)) T grad_share(int a, const int (&u)[P]) const
     {
