@@ -265,13 +265,14 @@ def test_log_signature_vjp_w8_d4_4096_streams(gpu):
     assert worst < 1e-6
 
 
-@pytest.mark.parametrize("w,d,batch", [(3, 4, 6), (8, 4, 5), (5, 5, 3), (4, 6, 3), (2, 7, 4)])
+@pytest.mark.parametrize("w,d,batch", [(3, 4, 6), (8, 4, 5), (5, 5, 3), (4, 6, 3), (2, 7, 4),
+                                       (3, 8, 2), (1, 5, 2), (4, 1, 3)])
 def test_ft_log_vjp_against_central_differences(gpu, w, d, batch):
     """rpy_dense_ft_log_vjp (reverse Horner chain of the log) against float64
     central differences (step 1e-6) of <H, log(S)> with the oracle's log, on
-    sampled coefficients of every degree; (4,6) and (2,7) take the
-    global-workspace path (the four float64 tensors exceed LDS).  The float32
-    call equals the float64 one to float32 rounding."""
+    sampled coefficients of every degree; (3,8) takes the global-workspace
+    path (its float64 arrays, 197 KB, exceed LDS), the others run in LDS.
+    The float32 call equals the float64 one to float32 rounding."""
     torch = gpu
     rng = np.random.default_rng(300 + 10 * w + d)
     tb = TensorBasis(w, d)
