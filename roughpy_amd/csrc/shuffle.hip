@@ -214,11 +214,24 @@ constexpr uint32_t padded(uint32_t n)
     return (n + v - 1) / v * v;
 }
 
-// LDS bytes for the staged pair, 0 if it does not fit the budget
+// The adjoint's arg in LDS: one pad word after every 32, so that reads at a
+// stride of a power of the width (a letter of u or v at one position) fall on
+// different banks of the 32 four-byte LDS banks rather than on one.
+RPY_FI uint32_t adj_pad(uint32_t i) { return i + (i >> 5); }
+
+template <typename T>
+constexpr uint32_t adj_arg_words(uint32_t n)
+{
+    return padded<T>(n + (n >> 5) + 1);
+}
+
+// LDS bytes for the adjoint's staged pair (op, padded arg), 0 if it does not
+// fit the budget
 template <typename T, int W, int D>
 constexpr size_t st_lds_bytes()
 {
-    const size_t b = 2 * size_t(padded<T>(cdb<W>(D + 1))) * sizeof(T);
+    const size_t b = (size_t(padded<T>(cdb<W>(D + 1))) + adj_arg_words<T>(cdb<W>(D + 1))) *
+                     sizeof(T);
     return b <= 160 * 1024 ? b : 0;
 }
 
@@ -477,12 +490,13 @@ RPY_FI uint32_t unrank_subset(uint32_t j, int k, int n)
 // sum over the k-subsets S of the m+k positions with ranks [sb, sb + sc)
 // (in increasing mask order) and v-high values vh in [vb, ve) of
 // op(v) arg(interleaving), v = vh W + a
-template <typename T, int W, int D>
+template <typename T, int W, int D, bool Pad>
 RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, uint32_t ve,
                  uint32_t sb = 0, uint32_t sc = ~0u)
 {
+    auto ar = [&](uint32_t i) { return A[Pad ? adj_pad(i) : i]; };
     const int n = m + k;
-    if (k == 0) return P[0] * A[db_rt<W, D>(m) + u];
+    if (k == 0) return P[0] * ar(db_rt<W, D>(m) + u);
     T acc = T(0);
     const uint32_t dbk = db_rt<W, D>(k);
     const uint32_t dbn = db_rt<W, D>(n);
@@ -569,8 +583,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                         constexpr uint32_t a1v = uint32_t(decltype(a1)::value);
                         chen::static_for<W>([&](auto a0) {
                             constexpr uint32_t a0v = uint32_t(decltype(a0)::value);
-                            acc = fmaT(P[ob + a1v * W + a0v], A[ab + a1v * wp[1] + a0v * wp[0]],
-                                       acc);
+                            acc = fmaT(P[ob + a1v * W + a0v], ar(ab + a1v * wp[1] + a0v * wp[0]), acc);
                         });
                     });
                     V = ((V | ~M2) + 1u) & M2;
@@ -580,7 +593,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 for (uint32_t vh = vb; vh < ve; ++vh) {
                     const uint32_t ab = dbn + U + V;
                     chen::static_for<W>([&](auto a) {
-                        acc = fmaT(P[ob + a], A[ab + uint32_t(decltype(a)::value) * wp[0]], acc);
+                        acc = fmaT(P[ob + a], ar(ab + uint32_t(decltype(a)::value) * wp[0]), acc);
                     });
                     V = ((V | ~M) + 1u) & M;
                     ob += uint32_t(W);
@@ -591,7 +604,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 const uint32_t ob = dbk + vh * W;
                 const uint32_t ab = dbn + U + V;
                 chen::static_for<W>([&](auto a) {
-                    acc = fmaT(P[ob + a], A[ab + uint32_t(decltype(a)::value) * wp[0]], acc);
+                    acc = fmaT(P[ob + a], ar(ab + uint32_t(decltype(a)::value) * wp[0]), acc);
                 });
                 // odometer over the v-high digits
                 bool carry = true;
@@ -618,7 +631,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
 }
 
 template <typename T, int W, int D, bool InLds>
-__global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
+__global__ __launch_bounds__(1024) void st_adj_tiled_kernel(T* __restrict__ out,
                                                           const T* __restrict__ op,
                                                           const T* __restrict__ arg,
                                                           int64_t batch, int64_t os, int64_t ps,
@@ -626,7 +639,7 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
 {
     constexpr uint32_t N = cdb<W>(D + 1);
     extern __shared__ __align__(16) unsigned char st_smem[];
-    T* part = reinterpret_cast<T*>(st_smem) + (InLds ? 2 * padded<T>(N) : 0);
+    T* part = reinterpret_cast<T*>(st_smem) + (InLds ? padded<T>(N) + adj_arg_words<T>(N) : 0);
     // light outputs, level m0 and up
     uint32_t n_light = 0;
     for (int m = plan.m0; m <= D; ++m) n_light += wpow_rt<W, D + 1>(m);
@@ -640,7 +653,8 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
             T* Ps = reinterpret_cast<T*>(st_smem);
             T* As = Ps + padded<T>(N);
             stage_lds(Ps, op + e * ps, N);
-            stage_lds(As, arg + e * as, N);
+            const T* a = arg + e * as;
+            for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) As[adj_pad(i)] = a[i];
             __syncthreads();
             P = Ps;
             A = As;
@@ -662,7 +676,7 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
                 }
                 T s;
                 if (k == 0) {
-                    s = adj_job<T, W, D>(P, A, m, u, 0, 0, 1);
+                    s = adj_job<T, W, D, InLds>(P, A, m, u, 0, 0, 1);
                 } else {
                     // part r: mask block r / vparts, v-high block r % vparts
                     const uint32_t vh_n = wpow_rt<W, D>(k - 1);
@@ -672,7 +686,7 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
                     const uint32_t vb = (r - ps * vparts) * c;
                     const uint32_t ve = vb + c < vh_n ? vb + c : vh_n;
                     const uint32_t sc = uint32_t(plan.scount[m][k]);
-                    s = vb < ve ? adj_job<T, W, D>(P, A, m, u, k, vb, ve, ps * sc, sc) : T(0);
+                    s = vb < ve ? adj_job<T, W, D, InLds>(P, A, m, u, k, vb, ve, ps * sc, sc) : T(0);
                 }
                 part[t] = s;
             } else {
@@ -684,7 +698,7 @@ __global__ __launch_bounds__(512) void st_adj_tiled_kernel(T* __restrict__ out,
                 }
                 T s = T(0);
                 for (int k = 0; m + k <= D; ++k)
-                    s += adj_job<T, W, D>(P, A, m, r, k, 0, k == 0 ? 1u : wpow_rt<W, D>(k - 1));
+                    s += adj_job<T, W, D, InLds>(P, A, m, r, k, 0, k == 0 ? 1u : wpow_rt<W, D>(k - 1));
                 o[db_rt<W, D>(m) + r] += s;
             }
         }
@@ -809,7 +823,10 @@ bool launch_st_adj_tiled(void* out, const void* op, const void* arg, int64_t bat
 {
     constexpr size_t staged = st_lds_bytes<T, W, D>();
     constexpr bool in_lds = staged > 0;
-    const int block = staged > 80 * 1024 ? 512 : 256;
+    // a pair that takes more than half the LDS runs alone on its CU: 16 waves
+    // then hide the gather's LDS latency ((4,6) float64: 6.00 -> 4.67 ms
+    // against 8 waves); smaller pairs share the CU and keep 4 waves each
+    const int block = staged > 80 * 1024 ? 1024 : 256;
     static const AdjPlan plan = make_adj_plan(W, D, block);
     const size_t lds = staged + (size_t(plan.n_heavy) * sizeof(T) + 15) / 16 * 16;
     if (lds > st_device_lds_limit()) return false;
