@@ -427,11 +427,13 @@ __global__ __launch_bounds__(256) void st_fma_tiled_kernel(T* __restrict__ out,
 //          op(v) arg(w), w the interleaving with v on S and u elsewhere
 // (position j = j-th letter from the end, weight W^j in the level index).
 // The work for an output of level m is sum_k C(m+k, k) W^k terms: thousands
-// for m <= 2, one for m = D.  Outputs whose work exceeds about twice the
-// per-task grain are "heavy": each (u, k) job is split over chunks of the v
-// range into sub-jobs whose partial sums land in LDS and are added in a fixed
-// order afterwards; "light" outputs get one thread each.  Every sum has a
-// fixed order, so results are deterministic.
+// for m <= 2, one for m = D.  A thread takes the W outputs u' c (c < W) of a
+// level m >= 1 together -- one op value per W multiply-adds, as the product's
+// letter tile -- and level 0 alone.  Units whose work exceeds about twice the
+// per-task grain are "heavy": each (unit, k) job is split over blocks of masks
+// and chunks of the v range into sub-jobs whose partial sums land in LDS and
+// are added in a fixed order afterwards; "light" units get one thread each.
+// Every sum has a fixed order, so results are deterministic.
 // ---------------------------------------------------------------------------
 
 constexpr int kAdjMaxParts = 4096;
@@ -439,13 +441,13 @@ constexpr int kAdjMaxParts = 4096;
 struct AdjPlan {
     int m0;                                              // heavy levels are m < m0
     int n_heavy;                                         // sub-jobs
-    int n_heavy_u;                                       // heavy outputs
+    int n_heavy_u;                                       // heavy units
     int parts[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1];  // parts[m][k]
     int chunk[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1];  // v-high values per part
     int scount[kMaxShuffleDepth + 1][kMaxShuffleDepth + 1]; // masks per part
-    int per_u[kMaxShuffleDepth + 1];                     // sub-jobs per heavy output of level m
+    int per_u[kMaxShuffleDepth + 1];                     // sub-jobs per heavy unit of level m
     int base[kMaxShuffleDepth + 2];                      // first sub-job of level m
-    int ubase[kMaxShuffleDepth + 2];                     // first heavy output of level m
+    int ubase[kMaxShuffleDepth + 2];                     // first heavy unit of level m
 };
 
 template <int W, int D>
@@ -487,17 +489,29 @@ RPY_FI uint32_t unrank_subset(uint32_t j, int k, int n)
     return S;
 }
 
-// sum over the k-subsets S of the m+k positions with ranks [sb, sb + sc)
+// Sum over the k-subsets S of the m+k positions with ranks [sb, sb + sc)
 // (in increasing mask order) and v-high values vh in [vb, ve) of
-// op(v) arg(interleaving), v = vh W + a
-template <typename T, int W, int D, bool Pad>
-RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, uint32_t ve,
-                 uint32_t sb = 0, uint32_t sc = ~0u)
+// op(v) arg(interleaving), v = vh W + a, for the TW outputs u0 + c, c < TW:
+// acc[c] += the terms of output u0 + c.  With TW = W, u0's last letter is 0
+// and the outputs share everything but that letter: for every S it sits at
+// the lowest position outside S (weight wc), so output u0 + c reads arg at
+// the offset of u0 plus c wc, and one op value serves W multiply-adds.  Each
+// acc[c] receives its output's terms in the order of the one-output job.
+template <typename T, int W, int D, bool Pad, int TW>
+RPY_FI void adj_job(T (&accs)[TW], const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb,
+                    uint32_t ve, uint32_t sb = 0, uint32_t sc = ~0u)
 {
     auto ar = [&](uint32_t i) { return A[Pad ? adj_pad(i) : i]; };
     const int n = m + k;
-    if (k == 0) return P[0] * ar(db_rt<W, D>(m) + u);
-    T acc = T(0);
+    if (k == 0) {
+        const T p0 = P[0];
+        chen::static_for<TW>([&](auto c) {
+            accs[c] = fmaT(p0, ar(db_rt<W, D>(m) + u + uint32_t(decltype(c)::value)), accs[c]);
+        });
+        return;
+    }
+    T acc[TW];
+    chen::static_for<TW>([&](auto c) { acc[c] = T(0); });
     const uint32_t dbk = db_rt<W, D>(k);
     const uint32_t dbn = db_rt<W, D>(n);
     const uint32_t limit = 1u << n;
@@ -505,10 +519,12 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
     for (uint32_t cnt = 0; cnt < sc && S < limit; ++cnt) {
         // U: the u letters on the complement of S (lowest position = last
         // letter); wp[i] = W^(position of the i-th lowest bit of S); for
-        // W = 2^b, M: the bit fields of S's positions above its lowest
+        // W = 2^b, M: the bit fields of S's positions above its lowest;
+        // wc: W^(lowest position outside S), u's last letter
         uint32_t U = 0;
         uint32_t wp[D];
         uint32_t M = 0;
+        uint32_t wc = 0;
         if constexpr ((W & (W - 1)) == 0) {
             // a level index holds letter j in bits [j b, j b + b): U is u
             // with an empty field opened at each position of S, in one pass
@@ -528,6 +544,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                     if (i > 0) M |= uint32_t(W - 1) << sh;
                 }
             });
+            if constexpr (TW > 1) wc = 1u << (uint32_t(__builtin_ctz(~S)) * b);
         } else {
             uint32_t ur = u;
             chen::static_for<D>([&](auto jc) {
@@ -546,6 +563,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                     t &= t - 1u;
                 }
             });
+            if constexpr (TW > 1) wc = wpow_rt<W, D>(__builtin_ctz(~S));
         }
         // v-high digits of vb and the matching offset
         uint32_t dv[D];
@@ -562,6 +580,13 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 }
             });
         }
+        // one op value times the tile's TW arg values
+        auto terms = [&](T pv, uint32_t ai) {
+            chen::static_for<TW>([&](auto c) {
+                constexpr uint32_t cv = uint32_t(decltype(c)::value);
+                acc[cv] = fmaT(pv, ar(ai + cv * wc), acc[cv]);
+            });
+        };
         if constexpr ((W & (W - 1)) == 0) {
             // W = 2^b: a level index holds letter j in bits [j b, j b + b), so
             // the v-high offset V is vh's digits deposited into the bit fields
@@ -569,8 +594,9 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
             // increment ((V | ~M) + 1) & M over those fields
             constexpr uint32_t b = uint32_t(__builtin_ctz(uint32_t(W)));
             uint32_t ob = dbk + vb * W;
-            // (float64 at W >= 4 measured slower unrolled: (4,6) 6.87 -> 7.15 ms)
-            constexpr bool kTwo = sizeof(T) == 4 || W * W <= 4;
+            // (float64 at W >= 4 measured slower unrolled: (4,6) 6.87 -> 7.15 ms;
+            // so did the float32 tiles, 2.49 -> 2.95 ms, out of registers)
+            constexpr bool kTwo = (sizeof(T) == 4 || W * W <= 4) && D >= 2 && TW == 1;
             if (kTwo && k >= 2 && vb % uint32_t(W) == 0 && ve % uint32_t(W) == 0) {
                 // v's last two letters unrolled (W^2 terms per step, in the
                 // same order as one letter at a time): the walk runs over
@@ -583,7 +609,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                         constexpr uint32_t a1v = uint32_t(decltype(a1)::value);
                         chen::static_for<W>([&](auto a0) {
                             constexpr uint32_t a0v = uint32_t(decltype(a0)::value);
-                            acc = fmaT(P[ob + a1v * W + a0v], ar(ab + a1v * wp[1] + a0v * wp[0]), acc);
+                            terms(P[ob + a1v * W + a0v], ab + a1v * wp[D >= 2 ? 1 : 0] + a0v * wp[0]);
                         });
                     });
                     V = ((V | ~M2) + 1u) & M2;
@@ -593,7 +619,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 for (uint32_t vh = vb; vh < ve; ++vh) {
                     const uint32_t ab = dbn + U + V;
                     chen::static_for<W>([&](auto a) {
-                        acc = fmaT(P[ob + a], ar(ab + uint32_t(decltype(a)::value) * wp[0]), acc);
+                        terms(P[ob + a], ab + uint32_t(decltype(a)::value) * wp[0]);
                     });
                     V = ((V | ~M) + 1u) & M;
                     ob += uint32_t(W);
@@ -604,7 +630,7 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
                 const uint32_t ob = dbk + vh * W;
                 const uint32_t ab = dbn + U + V;
                 chen::static_for<W>([&](auto a) {
-                    acc = fmaT(P[ob + a], ar(ab + uint32_t(decltype(a)::value) * wp[0]), acc);
+                    terms(P[ob + a], ab + uint32_t(decltype(a)::value) * wp[0]);
                 });
                 // odometer over the v-high digits
                 bool carry = true;
@@ -627,22 +653,42 @@ RPY_FI T adj_job(const T* P, const T* A, int m, uint32_t u, int k, uint32_t vb, 
         const uint32_t ripple = S + lowest;
         S = ripple | (((S ^ ripple) >> 2) / lowest);
     }
-    return acc;
+    chen::static_for<TW>([&](auto c) { accs[c] += acc[c]; });
+}
+
+// Outputs of level m >= 1 go in units of W (the tile u' c, c < W); level 0
+// is one unit of one output.
+template <int W>
+RPY_FI uint32_t adj_units(int m)
+{
+    uint32_t r = 1;
+    for (int j = 1; j < m; ++j) r *= uint32_t(W);
+    return r;
+}
+
+// Threads per workgroup: a float64 pair that takes more than half the LDS
+// runs alone on its CU, and 16 waves then hide the gather's LDS latency
+// ((4,6): 6.00 -> 4.67 ms against 8 waves); other pairs share the CU with 4
+// waves per workgroup.
+template <typename T, int W, int D>
+constexpr int adj_block()
+{
+    return (sizeof(T) == 8 && st_lds_bytes<T, W, D>() > 80 * 1024) ? 1024 : 256;
 }
 
 template <typename T, int W, int D, bool InLds>
-__global__ __launch_bounds__(1024) void st_adj_tiled_kernel(T* __restrict__ out,
-                                                          const T* __restrict__ op,
-                                                          const T* __restrict__ arg,
-                                                          int64_t batch, int64_t os, int64_t ps,
-                                                          int64_t as, AdjPlan plan)
+__global__ __launch_bounds__((adj_block<T, W, D>())) void st_adj_tiled_kernel(T* __restrict__ out,
+                                                           const T* __restrict__ op,
+                                                           const T* __restrict__ arg,
+                                                           int64_t batch, int64_t os, int64_t ps,
+                                                           int64_t as, AdjPlan plan)
 {
     constexpr uint32_t N = cdb<W>(D + 1);
     extern __shared__ __align__(16) unsigned char st_smem[];
     T* part = reinterpret_cast<T*>(st_smem) + (InLds ? padded<T>(N) + adj_arg_words<T>(N) : 0);
-    // light outputs, level m0 and up
+    // light units, level m0 and up
     uint32_t n_light = 0;
-    for (int m = plan.m0; m <= D; ++m) n_light += wpow_rt<W, D + 1>(m);
+    for (int m = plan.m0; m <= D; ++m) n_light += adj_units<W>(m);
     const uint32_t total = uint32_t(plan.n_heavy) + n_light;
     for (int64_t e = blockIdx.x; e < batch; e += gridDim.x) {
         T* o = out + e * os;
@@ -667,55 +713,90 @@ __global__ __launch_bounds__(1024) void st_adj_tiled_kernel(T* __restrict__ out,
                 int m = 0;
                 while (m + 1 < plan.m0 && t >= uint32_t(plan.base[m + 1])) ++m;
                 const uint32_t r0 = t - uint32_t(plan.base[m]);
-                const uint32_t u = r0 / uint32_t(plan.per_u[m]);
-                uint32_t r = r0 - u * uint32_t(plan.per_u[m]);
+                const uint32_t ui = r0 / uint32_t(plan.per_u[m]);
+                uint32_t r = r0 - ui * uint32_t(plan.per_u[m]);
                 int k = 0;
                 while (r >= uint32_t(plan.parts[m][k])) {
                     r -= uint32_t(plan.parts[m][k]);
                     ++k;
                 }
-                T s;
-                if (k == 0) {
-                    s = adj_job<T, W, D, InLds>(P, A, m, u, 0, 0, 1);
-                } else {
+                T s[W];
+                chen::static_for<W>([&](auto c) { s[c] = T(0); });
+                uint32_t vb = 0, ve = 1, sb = 0, sc = ~0u;
+                if (k > 0) {
                     // part r: mask block r / vparts, v-high block r % vparts
                     const uint32_t vh_n = wpow_rt<W, D>(k - 1);
                     const uint32_t c = uint32_t(plan.chunk[m][k]);
                     const uint32_t vparts = (vh_n + c - 1) / c;
-                    const uint32_t ps = r / vparts;
-                    const uint32_t vb = (r - ps * vparts) * c;
-                    const uint32_t ve = vb + c < vh_n ? vb + c : vh_n;
-                    const uint32_t sc = uint32_t(plan.scount[m][k]);
-                    s = vb < ve ? adj_job<T, W, D, InLds>(P, A, m, u, k, vb, ve, ps * sc, sc) : T(0);
+                    const uint32_t pb = r / vparts;
+                    vb = (r - pb * vparts) * c;
+                    ve = vb + c < vh_n ? vb + c : vh_n;
+                    sc = uint32_t(plan.scount[m][k]);
+                    sb = pb * sc;
                 }
-                part[t] = s;
+                if (vb < ve) {
+                    if (m == 0) {
+                        T s1[1] = {T(0)};
+                        adj_job<T, W, D, InLds, 1>(s1, P, A, 0, 0, k, vb, ve, sb, sc);
+                        s[0] = s1[0];
+                    } else {
+                        adj_job<T, W, D, InLds, W>(s, P, A, m, ui * uint32_t(W), k, vb, ve, sb,
+                                                   sc);
+                    }
+                }
+                chen::static_for<W>([&](auto c) { part[t * W + uint32_t(decltype(c)::value)] = s[c]; });
             } else {
                 uint32_t r = t - uint32_t(plan.n_heavy);
                 int m = plan.m0;
-                while (r >= wpow_rt<W, D + 1>(m)) {
-                    r -= wpow_rt<W, D + 1>(m);
+                while (r >= adj_units<W>(m)) {
+                    r -= adj_units<W>(m);
                     ++m;
                 }
-                T s = T(0);
-                for (int k = 0; m + k <= D; ++k)
-                    s += adj_job<T, W, D, InLds>(P, A, m, r, k, 0, k == 0 ? 1u : wpow_rt<W, D>(k - 1));
-                o[db_rt<W, D>(m) + r] += s;
+                if (m == 0) {
+                    T s[1] = {T(0)};
+                    for (int k = 0; k <= D; ++k) {
+                        T sk[1] = {T(0)};
+                        adj_job<T, W, D, InLds, 1>(sk, P, A, 0, 0, k, 0,
+                                                   k == 0 ? 1u : wpow_rt<W, D>(k - 1));
+                        s[0] += sk[0];
+                    }
+                    o[0] += s[0];
+                } else {
+                    T s[W];
+                    chen::static_for<W>([&](auto c) { s[c] = T(0); });
+                    for (int k = 0; m + k <= D; ++k) {
+                        T sk[W];
+                        chen::static_for<W>([&](auto c) { sk[c] = T(0); });
+                        adj_job<T, W, D, InLds, W>(sk, P, A, m, r * uint32_t(W), k, 0,
+                                                   k == 0 ? 1u : wpow_rt<W, D>(k - 1));
+                        chen::static_for<W>([&](auto c) { s[c] += sk[c]; });
+                    }
+                    T* oo = o + db_rt<W, D>(m) + r * uint32_t(W);
+                    chen::static_for<W>([&](auto c) { oo[uint32_t(decltype(c)::value)] += s[c]; });
+                }
             }
         }
         __syncthreads();
-        for (uint32_t hu = threadIdx.x; hu < uint32_t(plan.n_heavy_u); hu += blockDim.x) {
+        // heavy outputs: each (unit, letter) sums its sub-jobs in order
+        for (uint32_t hc = threadIdx.x; hc < uint32_t(plan.n_heavy_u) * W; hc += blockDim.x) {
+            const uint32_t hu = hc / uint32_t(W);
+            const uint32_t c = hc - hu * uint32_t(W);
             int m = 0;
             while (m + 1 < plan.m0 && hu >= uint32_t(plan.ubase[m + 1])) ++m;
-            const uint32_t u = hu - uint32_t(plan.ubase[m]);
-            const uint32_t p0 = uint32_t(plan.base[m]) + u * uint32_t(plan.per_u[m]);
+            if (m == 0 && c > 0) continue;
+            const uint32_t ui = hu - uint32_t(plan.ubase[m]);
+            const uint32_t p0 = uint32_t(plan.base[m]) + ui * uint32_t(plan.per_u[m]);
             T s = T(0);
-            for (int q = 0; q < plan.per_u[m]; ++q) s += part[p0 + q];
-            o[db_rt<W, D>(m) + u] += s;
+            for (int q = 0; q < plan.per_u[m]; ++q) s += part[(p0 + uint32_t(q)) * W + c];
+            o[m == 0 ? 0u : db_rt<W, D>(m) + ui * uint32_t(W) + c] += s;
         }
     }
 }
 
-// Splits the adjoint work of a (W, D) tensor for `block` threads.
+// Splits the adjoint work of a (W, D) tensor for `block` threads.  A unit is
+// the W outputs u' c of a level m >= 1 (one output at level 0); units whose
+// work exceeds about twice the grain are heavy and split into sub-jobs, each
+// with W partial sums in LDS (at most kAdjMaxParts values).
 inline AdjPlan make_adj_plan(int W, int D, int block)
 {
     auto binom = [](int a, int b) {
@@ -724,19 +805,21 @@ inline AdjPlan make_adj_plan(int W, int D, int block)
         return r;
     };
     auto wp = [&](int e) { return double(ipow(W, e)); };
+    auto units = [&](int m) { return m == 0 ? 1.0 : wp(m - 1); };
+    auto width = [&](int m) { return m == 0 ? 1.0 : double(W); };
     double total = 0;
     for (int n = 0; n <= D; ++n) total += wp(n) * double(int64_t(1) << n);
     double grain = std::max(64.0, total / (4.0 * block));
     AdjPlan p{};
     for (;;) {
         p = AdjPlan{};
-        // heavy levels: every level up to the last one whose output work
+        // heavy levels: every level up to the last one whose unit work
         // exceeds twice the grain (the work is not monotone in m)
         p.m0 = 0;
         for (int m = 0; m <= D; ++m) {
             double work = 0;
             for (int k = 0; m + k <= D; ++k) work += binom(m + k, k) * wp(k);
-            if (work > 2 * grain) p.m0 = m + 1;
+            if (work * width(m) > 2 * grain) p.m0 = m + 1;
         }
         int64_t nh = 0, nu = 0;
         for (int m = 0; m < p.m0; ++m) {
@@ -753,7 +836,8 @@ inline AdjPlan make_adj_plan(int W, int D, int block)
                     const double vh = wp(k - 1);
                     const double masks = binom(m + k, k);
                     const double target = std::min(
-                            masks * vh, std::max(1.0, std::ceil(masks * wp(k) / grain)));
+                            masks * vh,
+                            std::max(1.0, std::ceil(masks * wp(k) * width(m) / grain)));
                     scount = int(std::ceil(masks / std::min(masks, target)));
                     const int sparts = int(std::ceil(masks / scount));
                     const double vp = std::min(vh, std::max(1.0, std::ceil(target / sparts)));
@@ -766,14 +850,14 @@ inline AdjPlan make_adj_plan(int W, int D, int block)
                 per += parts;
             }
             p.per_u[m] = per;
-            nh += int64_t(per) * int64_t(ipow(W, m));
-            nu += int64_t(ipow(W, m));
+            nh += int64_t(per) * int64_t(units(m));
+            nu += int64_t(units(m));
         }
         p.base[p.m0] = int(nh);
         p.ubase[p.m0] = int(nu);
         p.n_heavy = int(nh);
         p.n_heavy_u = int(nu);
-        if (nh <= kAdjMaxParts) break;
+        if (nh * W <= kAdjMaxParts) break;
         grain *= 2;
     }
     return p;
@@ -823,12 +907,9 @@ bool launch_st_adj_tiled(void* out, const void* op, const void* arg, int64_t bat
 {
     constexpr size_t staged = st_lds_bytes<T, W, D>();
     constexpr bool in_lds = staged > 0;
-    // a pair that takes more than half the LDS runs alone on its CU: 16 waves
-    // then hide the gather's LDS latency ((4,6) float64: 6.00 -> 4.67 ms
-    // against 8 waves); smaller pairs share the CU and keep 4 waves each
-    const int block = staged > 80 * 1024 ? 1024 : 256;
+    constexpr int block = adj_block<T, W, D>();
     static const AdjPlan plan = make_adj_plan(W, D, block);
-    const size_t lds = staged + (size_t(plan.n_heavy) * sizeof(T) + 15) / 16 * 16;
+    const size_t lds = staged + (size_t(plan.n_heavy) * W * sizeof(T) + 15) / 16 * 16;
     if (lds > st_device_lds_limit()) return false;
     const uint32_t grid = uint32_t(std::min<int64_t>(batch, 16384));
     static std::atomic<bool> reserved{false};
