@@ -15,7 +15,7 @@ struct FastDiv {
     uint32_t d, m, s;
 };
 
-inline FastDiv make_fastdiv(uint32_t d)
+RPY_HD inline FastDiv make_fastdiv(uint32_t d)
 {
     uint32_t s = 0;
     while ((uint64_t(1) << s) < d) ++s;
