@@ -352,40 +352,59 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const size_t tel = size_t(n_streams) * size_t(tsize);
     const size_t elt = (dtype == RPY_F64) ? 8 : 4;
-    // sig (dtype) | sig widened to f64 (if needed) | log (f64), each segment
-    // starting on a 256-byte boundary
-    auto seg = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
-    const size_t sig_bytes = seg(tel * elt);
-    const size_t wide_bytes = dtype == RPY_F64 ? 0 : seg(tel * 8);
-    char* ws = nullptr;
-    RPY_HIP_CHECK(workspace_alloc((void**)&ws, sig_bytes + wide_bytes + seg(tel * 8), st));
-    void* sig = ws;
-    double* sig64 = (dtype == RPY_F64) ? (double*)ws : (double*)(ws + sig_bytes);
-    double* lg = (double*)(ws + sig_bytes + wide_bytes);
+    void* sig = nullptr;
+    RPY_HIP_CHECK(workspace_alloc(&sig, tel * elt, st));
     int rc = stream_signature_batch(sig, increments, offsets, n_streams, width, depth, dtype,
                                     stream);
-    rpy_tensor_basis tb{width, depth, nullptr};
-    bool logged = false;
-    if (!rc && dtype != RPY_F64) {
-        // float32 signatures: the float64 log widens them as it loads them;
-        // shapes without a fixed-shape log widen first
-        rc = dense_ft_log_from_f32(lg, (const float*)sig, &tb, n_streams, tsize, tsize, stream);
-        logged = rc == RPY_OK;
-        if (rc == RPY_ERR_UNSUPPORTED) {
-            rc = RPY_OK;
-            hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0, st,
-                               sig64, (const float*)sig, int64_t(tel));
-            if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
+    // the fixed shapes: log and projection in one kernel, the log kept in LDS
+    bool done = false;
+    if (!rc) {
+        const int frc = dense_log_t2l_fixed(out, sig, dtype, width, depth, t2l, n_streams, lsize,
+                                            tsize, stream);
+        if (frc == RPY_OK)
+            done = true;
+        else if (frc != RPY_ERR_UNSUPPORTED)
+            rc = frc;
+    }
+    if (!rc && !done) {
+        // sig widened to f64 (if needed) | log (f64), each segment starting on
+        // a 256-byte boundary
+        auto seg = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+        const size_t wide_bytes = dtype == RPY_F64 ? 0 : seg(tel * 8);
+        char* ws = nullptr;
+        hipError_t aerr = workspace_alloc((void**)&ws, wide_bytes + seg(tel * 8), st);
+        if (aerr != hipSuccess) {
+            rc = fail(RPY_ERR_HIP, "workspace allocation failed: %s", hipGetErrorString(aerr));
+        } else {
+            double* sig64 = (dtype == RPY_F64) ? (double*)sig : (double*)ws;
+            double* lg = (double*)(ws + wide_bytes);
+            rpy_tensor_basis tb{width, depth, nullptr};
+            bool logged = false;
+            if (dtype != RPY_F64) {
+                // float32 signatures: the float64 log widens them as it loads
+                // them; shapes without a fixed-shape log widen first
+                rc = dense_ft_log_from_f32(lg, (const float*)sig, &tb, n_streams, tsize, tsize,
+                                           stream);
+                logged = rc == RPY_OK;
+                if (rc == RPY_ERR_UNSUPPORTED) {
+                    rc = RPY_OK;
+                    hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0,
+                                       st, sig64, (const float*)sig, int64_t(tel));
+                    if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
+                }
+            }
+            if (!rc && !logged)
+                rc = dense_ft_log(lg, sig64, RPY_F64, &tb, {0, -1}, {0, -1}, n_streams, tsize,
+                                  tsize, stream);
+            // the projection writes the output (no clearing pass, no read of it)
+            if (!rc)
+                rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize,
+                                      stream, false);
+            hipError_t ferr = hipFreeAsync(ws, st);
+            if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
         }
     }
-    if (!rc && !logged)
-        rc = dense_ft_log(lg, sig64, RPY_F64, &tb, {0, -1}, {0, -1}, n_streams, tsize, tsize,
-                          stream);
-    // the projection writes the output (no clearing pass, no read of it)
-    if (!rc)
-        rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize, stream,
-                              false);
-    hipError_t ferr = hipFreeAsync(ws, st);
+    hipError_t ferr = hipFreeAsync(sig, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
     return rc;
 }

@@ -60,7 +60,7 @@ def test_shuffle_vs_reference_vectors(gpu, key):
     assert _level_err(adj, g["st_adj_mul"], w, d) <= tol
 
 
-@pytest.mark.parametrize("w,d", [(2, 9), (3, 5), (4, 6), (5, 4), (8, 3)])
+@pytest.mark.parametrize("w,d", [(2, 9), (3, 5), (4, 6), (5, 4), (8, 3), (6, 4), (7, 3), (3, 7)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_shuffle_batched_vs_oracle(gpu, rng, w, d, dtype):
     tb = rc.TensorBasis(w, d)
