@@ -513,9 +513,12 @@ typedef enum rpy_kernel_kind {
     RPY_KIND_LANE_CHUNK = 1,   /* 64 (stream, prefix) lanes per wave */
     RPY_KIND_STREAM_GROUP = 2, /* W^P lanes of one stream per workgroup */
     RPY_KIND_WAVE_VJP = 3,     /* backward: W^P lanes per stream, registers */
-    RPY_KIND_BLOCK_MERGE = 4   /* RPY_F32_COMPENSATED: segments of `block`
+    RPY_KIND_BLOCK_MERGE = 4,  /* RPY_F32_COMPENSATED: segments of `block`
                                 * rows on the compensated lanes, merged per
                                 * stream with Kahan sums */
+    RPY_KIND_STREAM_TABLE = 5  /* the stream group with the path partials of
+                                * a block's steps computed once per prefix
+                                * group into LDS */
 } rpy_kernel_kind;
 
 typedef struct rpy_kernel_variant {

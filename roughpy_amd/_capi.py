@@ -34,6 +34,7 @@ RPY_KIND_LANE_CHUNK = 1
 RPY_KIND_STREAM_GROUP = 2
 RPY_KIND_WAVE_VJP = 3
 RPY_KIND_BLOCK_MERGE = 4
+RPY_KIND_STREAM_TABLE = 5
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64

@@ -229,6 +229,37 @@ struct Lane {
         });
     }
 
+    // The path work of one step, for the lanes whose prefix shares
+    // u_0..u_{P-2}: writes the path partial of every subtree target M to
+    // pp[(M - P) * stride] (the value target M would compute) and advances
+    // the path levels.  Lanes that take these partials (step_pp) do no path
+    // work, and the result is the same.
+    template <typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void path_step(const Acc& acc, T* pp,
+                                                                 int stride)
+    {
+        static_for<D - P + 1>([&](auto i) {
+            constexpr int M = D - int(decltype(i)::value);
+            pp[(M - P) * stride] = path_partial<M>(acc);
+        });
+        static_for<P - 1>([&](auto i) {
+            constexpr int M = P - 1 - int(decltype(i)::value);
+            path_target<M>(acc);
+        });
+    }
+
+    // One Chen step of the subtree levels with the path partials given
+    // (pp[(M - P) * stride] for target M); the path levels are not advanced.
+    template <typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void step_pp(const Acc& acc, const T* pp,
+                                                               int stride)
+    {
+        static_for<D - P + 1>([&](auto i) {
+            constexpr int M = D - int(decltype(i)::value);
+            target_p<M>(acc, pp[(M - P) * stride]);
+        });
+    }
+
   private:
 
     // hi += t with Kahan's compensation c (the value is hi - c).  No term
@@ -280,7 +311,12 @@ struct Lane {
     template <int M, typename Acc>
     RPY_HD inline __attribute__((always_inline)) void target(const Acc& acc)
     {
-        const T p = path_partial<M>(acc);
+        target_p<M>(acc, path_partial<M>(acc));
+    }
+
+    template <int M, typename Acc>
+    RPY_HD inline __attribute__((always_inline)) void target_p(const Acc& acc, const T p)
+    {
         // K = P: the single word u
         if constexpr (M == P && CS > 0) {
             kahan(sub[0], csub[0], p * acc.template xu<1>(P - 1));

@@ -216,7 +216,12 @@ struct StreamCfg {
     static constexpr int kPer = (kRaw + NT - 1) / NT;
 };
 
-template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false>
+// PT: the path partials of each step are computed once per prefix group
+// u_0..u_{P-2} (W^(P-1) builder threads) for the whole block ahead of the
+// step loop and read from LDS by the lanes, which then do only their
+// subtree work (at P = 3 the path work is 9 of a lane-step's 48 operations).
+template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false,
+          bool PT = false>
 __global__ __launch_bounds__((StreamCfg<T, W, D, P, BLK>::NT))
 void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
                         const int64_t* __restrict__ begins, const int64_t* __restrict__ ends,
@@ -229,6 +234,9 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
     __shared__ __align__(16) T stage[2][kBlk * STEP];
     __shared__ int64_t s_row0, s_len;
     __shared__ int32_t s_stream;
+    constexpr int NG = int(ipow(W, P - 1));  // prefix groups u_0..u_{P-2}
+    constexpr int NPT = D - P + 1;            // subtree targets
+    __shared__ __align__(16) T ptab[PT ? kBlk * NPT * NG : 1];
 
     const int tid = threadIdx.x;
     const bool active = tid < NP;
@@ -298,6 +306,18 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
 
         chen::Lane<T, W, D, P, Comp> st;
         st.init_identity();
+        // PT: the path levels live in the builder threads (tid < NG, the
+        // prefix group tid with letter u_{P-1} = 0)
+        chen::Lane<T, W, D, P, Comp> pst;
+        int ug[P];
+        if constexpr (PT) {
+            pst.init_identity();
+            int t = (tid < NG ? tid : 0) * W;
+            for (int j = P - 1; j >= 0; --j) {
+                ug[j] = t % W;
+                t /= W;
+            }
+        }
         fetch(row0, len, 0);
         store(stage[0]);
         __syncthreads();
@@ -306,6 +326,27 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
             const bool more = t0 + kBlk < len;
             if (more) fetch(row0, len, t0 + kBlk);
             const T* blk = stage[buf];
+            if constexpr (PT) {
+                // rows past the end of the stream are zero: they leave the
+                // path levels unchanged
+                if (tid < NG) {
+#pragma unroll 1
+                    for (int tt = 0; tt < kBlk; ++tt) {
+                        const T* step = blk + tt * STEP;
+                        T raw[W];
+                        chen::static_for<W>(
+                                [&](auto a) { raw[a] = step[W * LS + int(decltype(a)::value)]; });
+                        const T* lt[P];
+                        chen::static_for<P>([&](auto j) {
+                            constexpr int jj = int(decltype(j)::value);
+                            lt[jj] = step + ug[jj] * LS;
+                        });
+                        pst.path_step(chen::UniformRow<T, W, D, P, LS>{raw, lt},
+                                      ptab + tt * NPT * NG + tid, NG);
+                    }
+                }
+                __syncthreads();
+            }
 #pragma unroll 1
             for (int tt = 0; tt < kBlk; ++tt) {
                 const T* step = blk + tt * STEP;
@@ -316,7 +357,11 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
                     constexpr int jj = int(decltype(j)::value);
                     lt[jj] = step + u[jj] * LS;
                 });
-                st.step(chen::UniformRow<T, W, D, P, LS>{raw, lt});
+                if constexpr (PT)
+                    st.step_pp(chen::UniformRow<T, W, D, P, LS>{raw, lt},
+                               ptab + tt * NPT * NG + q / W, NG);
+                else
+                    st.step(chen::UniformRow<T, W, D, P, LS>{raw, lt});
                 // keep each step's loads with its step: hoisting a whole
                 // unrolled block's reads runs out of registers
                 __builtin_amdgcn_sched_barrier(0);
@@ -336,11 +381,21 @@ void chen_stream_kernel(IO* __restrict__ out, const IO* __restrict__ incr,
                     ok[v] = IO(st.template sub_out<Sh::off(k) + int(decltype(v)::value)>());
                 });
             });
-            chen::static_for<P>([&](auto i) {
-                constexpr int k = int(decltype(i)::value);
-                constexpr int grp = int(ipow(W, P - k));
-                if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.template path_out<k>());
-            });
+            if constexpr (PT) {
+                // path levels: written by the builder of each prefix group
+                chen::static_for<P>([&](auto i) {
+                    constexpr int k = int(decltype(i)::value);
+                    constexpr int grp = int(ipow(W, P - 1 - k));
+                    if (tid < NG && tid % grp == 0)
+                        o[b.db[k] + tid / grp] = IO(pst.template path_out<k>());
+                });
+            } else {
+                chen::static_for<P>([&](auto i) {
+                    constexpr int k = int(decltype(i)::value);
+                    constexpr int grp = int(ipow(W, P - k));
+                    if (q % grp == 0) o[b.db[k] + q / grp] = IO(st.template path_out<k>());
+                });
+            }
         }
     }
 }
@@ -367,7 +422,8 @@ int launch_persistent(void* out, const void* incr, const int64_t* begins, const 
     return RPY_OK;
 }
 
-template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false>
+template <typename T, int W, int D, int P, typename IO = T, int BLK = 32, bool Comp = false,
+          bool PT = false>
 int launch_stream(void* out, const void* incr, const int64_t* begins, const int64_t* ends,
                   const int32_t* order, int64_t n_streams, const DevBasis& b, uint32_t* counter,
                   int flags, hipStream_t st)
@@ -377,10 +433,10 @@ int launch_stream(void* out, const void* incr, const int64_t* begins, const int6
     RPY_HIP_CHECK(hipGetDevice(&dev));
     RPY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RPY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, chen_stream_kernel<T, W, D, P, IO, BLK, Comp>, C::NT, 0));
+            &per_cu, chen_stream_kernel<T, W, D, P, IO, BLK, Comp, PT>, C::NT, 0));
     if (per_cu < 1) per_cu = 1;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_streams, int64_t(cus) * per_cu));
-    hipLaunchKernelGGL((chen_stream_kernel<T, W, D, P, IO, BLK, Comp>), dim3(uint32_t(grid)), dim3(C::NT),
+    hipLaunchKernelGGL((chen_stream_kernel<T, W, D, P, IO, BLK, Comp, PT>), dim3(uint32_t(grid)), dim3(C::NT),
                        0, st, (IO*)out, (const IO*)incr, begins, ends, order, n_streams, b.size(),
                        b, counter, flags);
     RPY_HIP_CHECK(hipGetLastError());

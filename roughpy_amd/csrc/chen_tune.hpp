@@ -49,6 +49,16 @@ void add_stream(std::vector<SigVariant>& v)
         v.push_back({W, D, RPY_F64, k, &launch_stream<double, W, D, P, double, BLK>});
 }
 
+// the stream kernel with the block's path partials tabulated per prefix group
+template <int W, int D, int P, int BLK>
+void add_stream_table(std::vector<SigVariant>& v)
+{
+    static_assert(ipow(W, P) >= 64 && ipow(W, P) <= 1024, "stream kernel needs 64..1024 lanes");
+    constexpr rpy_kernel_variant k{RPY_KIND_STREAM_TABLE, P, BLK, 0};
+    v.push_back({W, D, RPY_F32, k, &launch_stream<float, W, D, P, float, BLK, false, true>});
+    v.push_back({W, D, RPY_F64, k, &launch_stream<double, W, D, P, double, BLK, false, true>});
+}
+
 }  // namespace
 
 }  // namespace rpy_amd

@@ -16,6 +16,8 @@ SigVariantList sig_tune_variants_5_5()
         add_stream<5, 5, 3, 32>(v);
         add_stream<5, 5, 3, 64>(v);
         add_stream<5, 5, 4, 32>(v);
+        add_stream_table<5, 5, 3, 16>(v);
+        add_stream_table<5, 5, 3, 32>(v);
         return v;
     }();
     return {v.data(), v.size()};

@@ -54,7 +54,8 @@ DTYPES = {"f32": _capi.RPY_F32, "f64": _capi.RPY_F64, "f32c": _capi.RPY_F32_COMP
 DTYPE_NAMES = {v: k for k, v in DTYPES.items()}
 DTYPE_ENUM = {_capi.RPY_F32: "RPY_F32", _capi.RPY_F64: "RPY_F64",
               _capi.RPY_F32_COMPENSATED: "RPY_F32_COMPENSATED"}
-KIND_NAMES = {0: "generic", 1: "lane-chunk", 2: "stream-group", 3: "wave", 4: "block-merge"}
+KIND_NAMES = {0: "generic", 1: "lane-chunk", 2: "stream-group", 3: "wave", 4: "block-merge",
+              5: "stream-table"}
 SIG_TOL = {_capi.RPY_F64: 1e-12, _capi.RPY_F32: 1e-5, _capi.RPY_F32_COMPENSATED: 1e-5}
 VJP_TOL = {_capi.RPY_F64: 1e-10, _capi.RPY_F32: 1e-4}
 
