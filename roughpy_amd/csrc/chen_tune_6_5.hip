@@ -14,6 +14,8 @@ SigVariantList sig_tune_variants_6_5()
         add_stream<6, 5, 3, 16>(v);
         add_stream<6, 5, 3, 32>(v);
         add_stream<6, 5, 3, 64>(v);
+        add_stream_table<6, 5, 3, 16>(v);
+        add_stream_table<6, 5, 3, 32>(v);
         return v;
     }();
     return {v.data(), v.size()};
