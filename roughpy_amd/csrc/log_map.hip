@@ -26,8 +26,8 @@ constexpr size_t kLogMapLds = 128 * 1024;    // LDS per workgroup, at most
 // The log-signature's last two steps in one kernel: the float64 tensor log of
 // E signatures (the fixed-shape Horner of horner_fixed.hpp, its result kept
 // in LDS instead of written out), then their projection by the t2l matrix
-// from LDS.  The log
-// tensors never go to HBM, and the matrix is read from L2 once per E tensors.
+// from LDS.  The log tensors never go to HBM, and the matrix is read from L2
+// once per E tensors.
 // Each projected coefficient sums its row's entries in order, as
 // csr_map_lds_kernel does.
 // ---------------------------------------------------------------------------
@@ -124,7 +124,8 @@ bool launch_log_t2l_fixed(double* out, const void* arg, int64_t batch, int64_t o
 {
     constexpr size_t np = (size_t(fdb<W>(D)) + 15) / 16 * 16;
     constexpr size_t nfp = (size_t(fdb<W>(D + 1)) + 1) / 2 * 2;
-    // E = tensors per workgroup: the most whose slots fit LDS_LOG_MAP_BUDGET
+    // E = tensors per workgroup: the most (up to kLogMapE) whose slots fit
+    // kLogMapLds with the Horner accumulator and the staged levels
     constexpr size_t per = nfp * sizeof(double), base = 2 * np * sizeof(double);
     constexpr int E = base + kLogMapE * per <= kLogMapLds ? kLogMapE
                       : base + 2 * per <= kLogMapLds       ? 2
