@@ -18,9 +18,14 @@ namespace rpy_amd {
 
 namespace {
 
-constexpr int kLogMapBlock = 512;
-constexpr int kLogMapE = 4;                  // tensors per workgroup, at most
-constexpr size_t kLogMapLds = 128 * 1024;    // LDS per workgroup, at most
+constexpr int kLogMapE = 4;  // tensors per workgroup, at most
+
+// Threads and LDS per workgroup by depth: up to depth 4 one tensor per
+// workgroup of 256 threads, more workgroups per CU ((8,4) x 4096: 0.444 ->
+// 0.413 ms per log-signature call, float64); deeper shapes two tensors per
+// workgroup of 512 ((5,5) x 16384: 0.80 against 0.91 ms with one).
+constexpr int log_map_block(int D) { return D <= 4 ? 256 : 512; }
+constexpr size_t log_map_lds(int D) { return D <= 4 ? 64 * 1024 : 128 * 1024; }
 
 // ---------------------------------------------------------------------------
 // The log-signature's last two steps in one kernel: the float64 tensor log of
@@ -32,12 +37,13 @@ constexpr size_t kLogMapLds = 128 * 1024;    // LDS per workgroup, at most
 // csr_map_lds_kernel does.
 // ---------------------------------------------------------------------------
 template <int W, int D, typename Tin, int E>
-__global__ __launch_bounds__(kLogMapBlock) void log_t2l_fixed_kernel(
+__global__ __launch_bounds__(log_map_block(D)) void log_t2l_fixed_kernel(
         double* __restrict__ out, const Tin* __restrict__ arg, int64_t batch, int64_t os,
         int64_t as, const int64_t* __restrict__ offs, const int64_t* __restrict__ idx,
         const double* __restrict__ val, int64_t rows)
 {
     using T = double;
+    constexpr int kLogMapBlock = log_map_block(D);
     constexpr uint32_t ns = uint32_t(fdb<W>(D));      // levels 0..D-1
     constexpr uint32_t np = (ns + 15) / 16 * 16;
     constexpr uint32_t nf = uint32_t(fdb<W>(D + 1));  // the whole tensor
@@ -126,6 +132,7 @@ bool launch_log_t2l_fixed(double* out, const void* arg, int64_t batch, int64_t o
     constexpr size_t nfp = (size_t(fdb<W>(D + 1)) + 1) / 2 * 2;
     // E = tensors per workgroup: the most (up to kLogMapE) whose slots fit
     // kLogMapLds with the Horner accumulator and the staged levels
+    constexpr size_t kLogMapLds = log_map_lds(D);
     constexpr size_t per = nfp * sizeof(double), base = 2 * np * sizeof(double);
     constexpr int E = base + kLogMapE * per <= kLogMapLds ? kLogMapE
                       : base + 2 * per <= kLogMapLds       ? 2
@@ -143,7 +150,7 @@ bool launch_log_t2l_fixed(double* out, const void* arg, int64_t batch, int64_t o
         }
         const int64_t groups = (batch + E - 1) / E;
         hipLaunchKernelGGL(kern, dim3(uint32_t(std::min<int64_t>(groups, 8192))),
-                           dim3(kLogMapBlock), lds, st, out, (const Tin*)arg, batch, os, as,
+                           dim3(log_map_block(D)), lds, st, out, (const Tin*)arg, batch, os, as,
                            m->offsets, m->indices, (const double*)m->data, m->rows);
         return true;
     }
