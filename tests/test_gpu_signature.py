@@ -369,3 +369,35 @@ def test_sorted_and_unsorted_schedules_agree(gpu, rng):
     first = rs.signature_batch(inc[:offs[h]], offs[:h + 1], w, d)
     second = rs.signature_batch(inc[offs[h]:], offs[h:] - offs[h], w, d)
     assert_array_equal(whole, np.concatenate([first, second]))
+
+
+@pytest.mark.parametrize("w,d,n", [(3, 4, 9), (8, 4, 5), (5, 5, 7), (4, 6, 3)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fused_log_signature_equals_the_separate_launches(gpu, rng, w, d, n, dtype):
+    """For the fixed shapes the batched log-signature runs the float64 log and
+    the Hall projection as one kernel, the log kept in LDS (tensors per
+    workgroup: 4 at (3,4), 1 at (8,4), 2 at (5,5) and (4,6); the batches
+    leave a partial last group).  It equals the float64 log (rpy_dense_ft_log)
+    and the projection (rpy_dense_sparse_map) launched separately on the same
+    signatures, to float64 rounding (the separate projection multiplies each
+    term by its scale, 1)."""
+    torch = gpu
+    from roughpy_amd import _device
+    from roughpy_amd.bases import LieBasis
+    inc, offs = _uneven(rng, n, 1, 80, w, dtype)
+    got = np.asarray(rs.log_signature_batch(inc, offs, w, d))
+    tb, lb = rc.TensorBasis(w, d), LieBasis(w, d)
+    size = tb.size()
+    sig = np.asarray(rs.signature_batch(inc, offs, w, d), dtype=np.float64)  # exact widening
+    S = torch.from_numpy(sig).cuda()
+    lg = torch.empty_like(S)
+    L, st = _capi.lib(), _device.stream_handle()
+    full = _capi.DegreeBoundsC(0, -1)
+    _capi.check(L.rpy_dense_ft_log(lg.data_ptr(), S.data_ptr(), _capi.RPY_F64, tb.c_struct(),
+                                   full, full, n, size, size, 1, 1, st))
+    out = torch.zeros((n, lb.size()), dtype=torch.float64, device="cuda")
+    t2l = lb.get_t2l_matrix(np.float64)
+    _capi.check(L.rpy_dense_sparse_map(out.data_ptr(), lg.data_ptr(), t2l.device_struct(),
+                                       _capi.RPY_F64, 1.0, n, lb.size(), size, 1, 1, st))
+    ref = out.cpu().numpy()
+    assert_allclose(got, ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()))
