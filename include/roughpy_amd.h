@@ -338,6 +338,20 @@ int rpy_stream_log_signature_batch(double *out, const void *increments,
                                    const rpy_sparse_matrix *t2l,
                                    void *stream);
 
+/* The last two steps of rpy_stream_log_signature_batch on signatures the
+ * caller already holds (float32 or float64, batch x tensor size, rows
+ * `sig_stride` apart): out (float64, batch x lie_size, rows `out_stride`
+ * apart) = t2l(log(sig)), the log in float64.  For the fixed shapes the log
+ * stays in LDS (one kernel); other shapes run the log and the projection as
+ * two launches.  No reference counterpart as one call: it is
+ * dense_ft_log (roughpy/compute/_src/dense_intermediate.cpp:260-280)
+ * followed by the t2l map (lite_context.h:899-902), for callers that keep
+ * the signatures, such as the log-signature backward. */
+int rpy_dense_log_to_lie(double *out, const void *sig, rpy_dtype dtype,
+                         int32_t width, int32_t depth, const rpy_sparse_matrix *t2l,
+                         int64_t batch, int64_t out_stride, int64_t sig_stride,
+                         void *stream);
+
 /* ---- dyadic signature cache (device) ------------------------------------ */
 
 /* Elements (of the cache dtype) of a dyadic cache storing node signatures of

@@ -112,6 +112,8 @@ SIGNATURES = {
                                                           ctypes.c_int, _vp]),
     "rpy_stream_log_signature_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32,
                                                       ctypes.c_int, _SM, _vp]),
+    "rpy_dense_log_to_lie": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i32, _i32, _SM, _i64, _i64,
+                                            _i64, _vp]),
     "rpy_param_to_resolution": (_i32, [_dbl]),
     "rpy_to_dyadic_intervals": (ctypes.c_int, [_dbl, _dbl, _i32, _p_i64,
                                                ctypes.POINTER(_i32), _i32,

@@ -164,14 +164,14 @@ def _make_logsig_fn():
             _capi.check(L.rpy_stream_signature_batch(
                 S.data_ptr(), inc.data_ptr(), offs.data_ptr(), n, width, depth, _code(inc),
                 _device.stream_handle()))
-            S64 = S if S.dtype == torch.float64 else S.double()
-            logS = torch.empty((n, size), dtype=torch.float64, device=inc.device)
-            B = _capi.DegreeBoundsC
-            _capi.check(L.rpy_dense_ft_log(
-                logS.data_ptr(), S64.data_ptr(), _capi.RPY_F64, tb.c_struct(), B(0, -1), B(0, -1),
-                n, size, size, 1, 1, _device.stream_handle()))
+            # the float64 log and the Hall projection (one kernel for the
+            # fixed shapes, the log kept in LDS; float32 signatures are
+            # widened as they load)
             t2l = lie_basis.get_t2l_matrix(np.float64)
-            out = _apply_map(t2l, logS)
+            out = torch.empty((n, lie_basis.size()), dtype=torch.float64, device=inc.device)
+            _capi.check(L.rpy_dense_log_to_lie(
+                out.data_ptr(), S.data_ptr(), _code(inc), width, depth, t2l.device_struct(), n,
+                lie_basis.size(), size, _device.stream_handle()))
             ctx.save_for_backward(inc, offs, S)
             ctx.args = (width, depth, lie_basis, tb)
             return out

@@ -335,6 +335,66 @@ int rpy_stream_signature_batch_noalloc(void* out, const void* increments,
                                           dtype, stream);
 }
 
+namespace {
+
+// out = t2l(log(sig)) for a batch of signatures (float32 or float64): one
+// fused kernel for the fixed shapes, else the float64 log (widening float32
+// first where no fixed-shape log exists) and the projection.
+int log_to_lie(double* out, const void* sig, rpy_dtype dtype, int32_t width, int32_t depth,
+               const rpy_sparse_matrix* t2l, int64_t batch, int64_t lsize, int64_t tsize,
+               int64_t os, int64_t ss, hipStream_t st)
+{
+    void* stream = reinterpret_cast<void*>(st);
+    // compensated float32 signatures are float32 data
+    const rpy_dtype store = dtype == RPY_F64 ? RPY_F64 : RPY_F32;
+    const int frc =
+            dense_log_t2l_fixed(out, sig, store, width, depth, t2l, batch, os, ss, stream);
+    if (frc != RPY_ERR_UNSUPPORTED) return frc;
+    const size_t tel = size_t(batch) * size_t(tsize);
+    // sig widened to f64 (if needed) | log (f64), each segment starting on
+    // a 256-byte boundary
+    auto seg = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    const size_t wide_bytes = dtype == RPY_F64 ? 0 : seg(tel * 8);
+    char* ws = nullptr;
+    RPY_HIP_CHECK(workspace_alloc((void**)&ws, wide_bytes + seg(tel * 8), st));
+    int rc = RPY_OK;
+    double* lg = (double*)(ws + wide_bytes);
+    rpy_tensor_basis tb{width, depth, nullptr};
+    bool logged = false;
+    const double* sig64 = (const double*)sig;
+    int64_t s64 = ss;
+    if (dtype != RPY_F64) {
+        // float32 signatures: the float64 log widens them as it loads them;
+        // shapes without a fixed-shape log widen first
+        rc = dense_ft_log_from_f32(lg, (const float*)sig, &tb, batch, tsize, ss, stream);
+        logged = rc == RPY_OK;
+        if (rc == RPY_ERR_UNSUPPORTED) {
+            rc = RPY_OK;
+            double* wide = (double*)ws;
+            if (ss == tsize) {
+                hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0, st,
+                                   wide, (const float*)sig, int64_t(tel));
+            } else {
+                for (int64_t e = 0; e < batch && rc == RPY_OK; ++e)
+                    hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(tsize)), dim3(256), 0, st,
+                                       wide + e * tsize, (const float*)sig + e * ss, tsize);
+            }
+            if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
+            sig64 = wide;
+            s64 = tsize;
+        }
+    }
+    if (!rc && !logged)
+        rc = dense_ft_log(lg, sig64, RPY_F64, &tb, {0, -1}, {0, -1}, batch, tsize, s64, stream);
+    // the projection writes the output (no clearing pass, no read of it)
+    if (!rc) rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, batch, os, tsize, stream, false);
+    hipError_t ferr = hipFreeAsync(ws, st);
+    if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
+    return rc;
+}
+
+}  // namespace
+
 int rpy_stream_log_signature_batch(double* out, const void* increments, const int64_t* offsets,
                                    int64_t n_streams, int32_t width, int32_t depth,
                                    rpy_dtype dtype, const rpy_sparse_matrix* t2l, void* stream)
@@ -356,57 +416,33 @@ int rpy_stream_log_signature_batch(double* out, const void* increments, const in
     RPY_HIP_CHECK(workspace_alloc(&sig, tel * elt, st));
     int rc = stream_signature_batch(sig, increments, offsets, n_streams, width, depth, dtype,
                                     stream);
-    // the fixed shapes: log and projection in one kernel, the log kept in LDS
-    bool done = false;
-    if (!rc) {
-        const int frc = dense_log_t2l_fixed(out, sig, dtype, width, depth, t2l, n_streams, lsize,
-                                            tsize, stream);
-        if (frc == RPY_OK)
-            done = true;
-        else if (frc != RPY_ERR_UNSUPPORTED)
-            rc = frc;
-    }
-    if (!rc && !done) {
-        // sig widened to f64 (if needed) | log (f64), each segment starting on
-        // a 256-byte boundary
-        auto seg = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
-        const size_t wide_bytes = dtype == RPY_F64 ? 0 : seg(tel * 8);
-        char* ws = nullptr;
-        hipError_t aerr = workspace_alloc((void**)&ws, wide_bytes + seg(tel * 8), st);
-        if (aerr != hipSuccess) {
-            rc = fail(RPY_ERR_HIP, "workspace allocation failed: %s", hipGetErrorString(aerr));
-        } else {
-            double* sig64 = (dtype == RPY_F64) ? (double*)sig : (double*)ws;
-            double* lg = (double*)(ws + wide_bytes);
-            rpy_tensor_basis tb{width, depth, nullptr};
-            bool logged = false;
-            if (dtype != RPY_F64) {
-                // float32 signatures: the float64 log widens them as it loads
-                // them; shapes without a fixed-shape log widen first
-                rc = dense_ft_log_from_f32(lg, (const float*)sig, &tb, n_streams, tsize, tsize,
-                                           stream);
-                logged = rc == RPY_OK;
-                if (rc == RPY_ERR_UNSUPPORTED) {
-                    rc = RPY_OK;
-                    hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(int64_t(tel))), dim3(256), 0,
-                                       st, sig64, (const float*)sig, int64_t(tel));
-                    if (hipGetLastError() != hipSuccess) rc = fail(RPY_ERR_HIP, "widen launch failed");
-                }
-            }
-            if (!rc && !logged)
-                rc = dense_ft_log(lg, sig64, RPY_F64, &tb, {0, -1}, {0, -1}, n_streams, tsize,
-                                  tsize, stream);
-            // the projection writes the output (no clearing pass, no read of it)
-            if (!rc)
-                rc = dense_sparse_map(out, lg, t2l, RPY_F64, 1.0, n_streams, lsize, tsize,
-                                      stream, false);
-            hipError_t ferr = hipFreeAsync(ws, st);
-            if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
-        }
-    }
+    if (!rc)
+        rc = log_to_lie(out, sig, dtype, width, depth, t2l, n_streams, lsize, tsize, lsize,
+                        tsize, st);
     hipError_t ferr = hipFreeAsync(sig, st);
     if (!rc && ferr != hipSuccess) rc = fail(RPY_ERR_HIP, "hipFreeAsync failed");
     return rc;
+}
+
+int rpy_dense_log_to_lie(double* out, const void* sig, rpy_dtype dtype, int32_t width,
+                         int32_t depth, const rpy_sparse_matrix* t2l, int64_t batch,
+                         int64_t out_stride, int64_t sig_stride, void* stream)
+{
+    if (!t2l) return fail(RPY_ERR_INVALID_ARGUMENT, "null t2l matrix");
+    if (dtype != RPY_F32 && dtype != RPY_F64) return fail(RPY_ERR_DTYPE, "unsupported dtype");
+    const int64_t tsize = tensor_size_checked(width, depth);
+    const int64_t lsize = (width >= 1 && depth >= 0) ? lie_size(width, depth) : -1;
+    if (tsize < 0 || lsize < 0) return fail(RPY_ERR_BASIS, "invalid width/depth");
+    if (t2l->rows != lsize || t2l->cols != tsize)
+        return fail(RPY_ERR_SHAPE, "t2l matrix must be %lld x %lld", (long long)lsize,
+                    (long long)tsize);
+    if (batch < 0) return fail(RPY_ERR_SHAPE, "negative batch");
+    if (batch == 0) return RPY_OK;
+    if (!out || !sig) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    if (batch > 1 && (out_stride < lsize || sig_stride < tsize))
+        return fail(RPY_ERR_SHAPE, "batch strides smaller than the Lie and tensor sizes");
+    return log_to_lie(out, sig, dtype, width, depth, t2l, batch, lsize, tsize, out_stride,
+                      sig_stride, reinterpret_cast<hipStream_t>(stream));
 }
 
 // Test hook (not part of the compute surface): runs the lane decomposition

@@ -401,3 +401,33 @@ def test_fused_log_signature_equals_the_separate_launches(gpu, rng, w, d, n, dty
                                        _capi.RPY_F64, 1.0, n, lb.size(), size, 1, 1, st))
     ref = out.cpu().numpy()
     assert_allclose(got, ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("w,d", [(5, 5), (2, 9)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_log_to_lie_of_held_signatures(gpu, rng, w, d, dtype):
+    """rpy_dense_log_to_lie on signatures the caller holds, with rows padded
+    (row strides above the tensor and Lie sizes), equals the batched
+    log-signature of the same streams: the fused kernel at (5,5), the
+    separate log (widening float32 first) and projection at (2,9)."""
+    torch = gpu
+    from roughpy_amd import _device
+    from roughpy_amd.bases import LieBasis
+    n = 6
+    inc, offs = _uneven(rng, n, 1, 50, w, dtype)
+    ref = np.asarray(rs.log_signature_batch(inc, offs, w, d))
+    sig = np.asarray(rs.signature_batch(inc, offs, w, d))
+    tb, lb = rc.TensorBasis(w, d), LieBasis(w, d)
+    size, lsize = tb.size(), lb.size()
+    S = torch.zeros((n, size + 3), dtype=torch.float64 if dtype == np.float64 else torch.float32,
+                    device="cuda")
+    S[:, :size] = torch.from_numpy(sig).cuda()
+    out = torch.full((n, lsize + 5), float("nan"), dtype=torch.float64, device="cuda")
+    code = _capi.RPY_F64 if dtype == np.float64 else _capi.RPY_F32
+    L = _capi.lib()
+    _capi.check(L.rpy_dense_log_to_lie(out.data_ptr(), S.data_ptr(), code, w, d,
+                                       lb.get_t2l_matrix(np.float64).device_struct(), n,
+                                       lsize + 5, size + 3, _device.stream_handle()))
+    got = out.cpu().numpy()
+    assert np.isnan(got[:, lsize:]).all()
+    assert_allclose(got[:, :lsize], ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()))
