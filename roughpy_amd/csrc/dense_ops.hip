@@ -1716,10 +1716,11 @@ int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis*
     if ((rc = check_batch(batch, {os, as}, b.size()))) return rc;
     if (batch == 0) return RPY_OK;
     if (!out || !arg) return fail(RPY_ERR_INVALID_ARGUMENT, "null data pointer");
+    // no error text for the missing shape: the caller falls back to widening
+    // first, and a successful fallback must leave no message behind
     auto f = find_horner_fixed<double, Horner::Log, float>(b.width, b.depth);
     if (!f || !f(out, nullptr, arg, batch, os, 0, as, as_stream(stream)))
-        return fail(RPY_ERR_UNSUPPORTED, "no fixed-shape log for width %d depth %d", b.width,
-                    b.depth);
+        return RPY_ERR_UNSUPPORTED;
     RPY_HIP_CHECK(hipGetLastError());
     return RPY_OK;
 }

@@ -41,12 +41,14 @@ int dense_ft_mul_new(void* out, const void* lhs, const void* rhs, rpy_dtype dtyp
                      int64_t rs, void* stream);
 
 // The float64 tensor log of float32 tensors (the widening fused into the
-// kernel's loads); RPY_ERR_UNSUPPORTED for shapes without a fixed-shape kernel.
+// kernel's loads); RPY_ERR_UNSUPPORTED, with no error text set, for shapes
+// without a fixed-shape kernel.
 int dense_ft_log_from_f32(double* out, const float* arg, const rpy_tensor_basis* basis,
                           int64_t batch, int64_t os, int64_t as, void* stream);
 // The float64 log of float32 or float64 signatures projected by t2l (rows =
 // the Lie dimension, columns = the tensor size) in one kernel, the log never
-// leaving LDS; RPY_ERR_UNSUPPORTED for shapes without a fused kernel.
+// leaving LDS; RPY_ERR_UNSUPPORTED, with no error text set, for shapes without
+// a fused kernel.
 int dense_log_t2l_fixed(double* out, const void* sig, rpy_dtype sig_dtype, int width, int depth,
                         const rpy_sparse_matrix* t2l, int64_t batch, int64_t os, int64_t as,
                         void* stream);

@@ -425,9 +425,14 @@ def test_log_to_lie_of_held_signatures(gpu, rng, w, d, dtype):
     out = torch.full((n, lsize + 5), float("nan"), dtype=torch.float64, device="cuda")
     code = _capi.RPY_F64 if dtype == np.float64 else _capi.RPY_F32
     L = _capi.lib()
-    _capi.check(L.rpy_dense_log_to_lie(out.data_ptr(), S.data_ptr(), code, w, d,
-                                       lb.get_t2l_matrix(np.float64).device_struct(), n,
+    t2l = lb.get_t2l_matrix(np.float64).device_struct()
+    # a known error text first: a successful call (the fallback included)
+    # leaves it in place
+    assert L.rpy_dense_log_to_lie(0, 0, code, w, d, t2l, -1, 0, 0, None) != _capi.RPY_OK
+    assert L.rpy_last_error() == b"negative batch"
+    _capi.check(L.rpy_dense_log_to_lie(out.data_ptr(), S.data_ptr(), code, w, d, t2l, n,
                                        lsize + 5, size + 3, _device.stream_handle()))
+    assert L.rpy_last_error() == b"negative batch"
     got = out.cpu().numpy()
     assert np.isnan(got[:, lsize:]).all()
     assert_allclose(got[:, :lsize], ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()))
