@@ -367,6 +367,31 @@ def test_error_codes_map_to_exceptions():
     assert rc == _capi.RPY_ERR_SHAPE
 
 
+def test_log_to_lie_checks_arguments_on_the_host():
+    """rpy_dense_log_to_lie rejects bad arguments before any device work: a
+    missing or mis-shaped t2l matrix (width 2, depth 3: 5 x 15), an unknown
+    dtype, an invalid basis, a negative batch, null data and short row
+    strides; an empty batch is a no-op."""
+    f = _capi.lib().rpy_dense_log_to_lie
+    t2l = _capi.SparseMatrixC(5, 15, 0, None, None, None, 1)
+    wrong = _capi.SparseMatrixC(5, 14, 0, None, None, None, 1)
+    ref = ctypes.byref
+
+    def call(m=t2l, dt=_capi.RPY_F64, w=2, d=3, n=2, os=5, ss=15, out=None, sig=None):
+        return f(out, sig, dt, w, d, None if m is None else ref(m), n, os, ss, None)
+
+    assert call(m=None) == _capi.RPY_ERR_INVALID_ARGUMENT
+    assert call(dt=7) == _capi.RPY_ERR_DTYPE
+    assert call(w=0) == _capi.RPY_ERR_BASIS
+    assert call(m=wrong) == _capi.RPY_ERR_SHAPE
+    assert call(n=-1) == _capi.RPY_ERR_SHAPE
+    assert call(n=0) == _capi.RPY_OK
+    assert call() == _capi.RPY_ERR_INVALID_ARGUMENT
+    # strides are checked before anything is read through the (dummy) pointers
+    assert call(os=4, out=8, sig=8) == _capi.RPY_ERR_SHAPE
+    assert call(ss=14, out=8, sig=8, dt=_capi.RPY_F32) == _capi.RPY_ERR_SHAPE
+
+
 def test_query_kernel_dyadic_cover_matches_reference_dissection(rng):
     """The query kernel's greedy cover of [A, B) (leaf units) equals
     to_dyadic_intervals of the clopen [A 2^-r, B 2^-r) (oracle restatement
